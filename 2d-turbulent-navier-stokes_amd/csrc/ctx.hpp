@@ -1,0 +1,97 @@
+// Library context and device views shared by the routine-level kernels
+// (routines.hip), the fused hot path (fused.hip) and the C ABI (api.cpp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "mesh_prep.hpp"
+#include "phys.hpp"
+#include "tdt.h"
+
+namespace tdt {
+
+constexpr int kBlock = 256;
+// Fused kernels per iteration: pre-pass, dissipation+viscous, stages 1..4.
+constexpr int kNumFusedKernels = 6;
+
+// Device view of the mesh, geometry and routine-level (reference-layout) arrays.
+// Internal ids: cells 0..n-1 (renumbered), ghosts n..n+g-1 (bc.in order),
+// nodes 0..m-1.  Component-major storage: a[k*stride + id].
+struct Dev {
+    int n, g, m, nt;          // nt = n + g
+    const int4 *nc4;          // [n] neighbours across faces 1..4
+    const int4 *nod4;         // [n] nodes 1..4
+    const double *xn, *yn;    // [m] metres
+    const int *gparent, *gside, *gtype, *gopp;  // [g]
+    const int *nptr, *nlist;  // node gather lists
+    const int4 *novr;         // [m]
+    double *xcc, *ycc, *vol;  // [nt]
+    double *dl;               // [n]
+    double *sc;               // [8][n] SC1X SC1Y SC2X SC2Y SC3X SC3Y SC4X SC4Y
+    double *c;                // [5][nt]  C1..C5
+    double *cold;             // [5][n]
+    double *u, *v, *t, *p, *dt;  // [nt]
+    double *f, *gf;           // [5][nt]  F1..F5, G1..G5
+    double *r;                // [5][n]
+    double *d2q;              // [5][nt]
+    double *d24;              // [5][n]
+    double *anu, *anurt;      // [nt]
+    double *vis;              // [4][n]  VISU VISV VIST VISR
+    double *src3, *src5;      // [n]
+    double *nodeavg;          // [5][m]  UN VN TN RHON RN
+    double *scal;             // [8]  0: DTMIN (as bits for atomic min) 1: RTOL ...
+    double *part;             // monitor partials
+    int *flags;               // [4]  0: non-finite flag
+};
+
+struct FusedBufs;  // fused.hip
+
+}  // namespace tdt
+
+struct tdt_ctx {
+    std::string err;
+    int precision = TDT_FP64;
+    int device = 0;
+    tdt_params params{};
+    tdt::PreparedMesh pm;
+    tdt::Consts<double> K{};
+    tdt::Consts<float> Kf{};
+    double rinit = 0.0;
+    double params_rtred = 0.0;
+    hipStream_t stream = nullptr;
+    tdt::Dev D{};
+    std::vector<void *> allocs;
+    tdt::FusedBufs *fused = nullptr;
+    int profiling = 0;
+    double ktimes[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+};
+
+namespace tdt {
+
+// Launch helpers implemented in routines.hip.
+hipError_t launch_geometry(tdt_ctx *c, int iaxis);
+hipError_t launch_timestep(tdt_ctx *c);
+hipError_t launch_dissipation(tdt_ctx *c);
+hipError_t launch_primitives(tdt_ctx *c);
+hipError_t launch_boundary(tdt_ctx *c);
+hipError_t launch_viscous(tdt_ctx *c);
+hipError_t launch_source(tdt_ctx *c);
+hipError_t launch_residual(tdt_ctx *c);
+hipError_t launch_update(tdt_ctx *c, double alpha);
+// L2 (out[0..4]) and Linf (out[5..9]) of C - COLD, flags[0] |= non-finite.
+hipError_t launch_monitor(tdt_ctx *c, int save_old, double *out);
+hipError_t launch_cold_start(tdt_ctx *c, double rho, double ru, double rv, double re,
+                             double rr);
+hipError_t launch_copy_cold(tdt_ctx *c);
+
+// Fused path (fused.hip).
+int fused_setup(tdt_ctx *c);
+void fused_free(tdt_ctx *c);
+int fused_iterate(tdt_ctx *c, int niter, double *l2, double *linf, int *first_nonfinite);
+
+inline unsigned nblocks(int count) { return (unsigned)((count + kBlock - 1) / kBlock); }
+
+}  // namespace tdt

@@ -1,0 +1,324 @@
+// Device-side physics of the residual + RK4 path.  Each function restates one
+// piece of the reference solver with the reference's evaluation order
+// (Fortran: left to right within a precedence level, parentheses honoured).
+// With R = double and -ffp-contract=off the arithmetic is bit-identical to the
+// reference except where libm (pow, tanh) differs from the device math
+// library.  R = float is the FP32-mixed arithmetic.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace tdt {
+
+// Model and scheme constants (constants.f90:15-45).
+constexpr double kCmu = 0.09, kAlphaTurb = 0.00052, kBetaTurb = 0.2;
+constexpr double kConst1 = 21.479, kConst2 = 21.394, kConst3 = 1.5;
+constexpr double kSuth = 0.00000011848;
+constexpr double kC02 = 0.25, kC04 = 0.00390625;
+constexpr double kCflTurb = 0.5;
+constexpr double kDiv23 = 2.0 / 3.0;
+constexpr double kYaxisThresh = 0.001, kAmuThresh = 0.001;
+constexpr double kBlExponent = 1.0 / 7.0;
+
+// Flow constants derived once on the host with the reference's expressions.
+template <class R>
+struct Consts {
+    R gama, cfl, prandtl, prandtlt, t0, pamb, pinf, deltabl;
+    R rr, cv, cp, s0, tinf;
+    R rtol, rtol_rtred;        // RTOLERANCE and RTOLERANCE*RTRED
+    R suth_sqrt_amol;          // SUTH_COEF*DSQRT(AMOL)
+    R inv_1mg;                 // 1/(1-GAMA)
+    R sqcmu;                   // DSQRT(2*DSQRT(CMU))
+    R bl_u;                    // 2*AMINF*DSQRT(GAMA*RR*TINF)
+    R gamma_rr;                // GAMA*RR
+    int local, npseudo, iaxis, pamb_set;
+};
+
+template <class R>
+__device__ __forceinline__ R dmax(R a, R b) { return a > b ? a : b; }
+template <class R>
+__device__ __forceinline__ R dmin(R a, R b) { return a < b ? a : b; }
+
+__device__ __forceinline__ double tpow(double a, double b) { return pow(a, b); }
+__device__ __forceinline__ float tpow(float a, float b) { return powf(a, b); }
+__device__ __forceinline__ double ttanh(double a) { return tanh(a); }
+__device__ __forceinline__ float ttanh(float a) { return tanhf(a); }
+__device__ __forceinline__ double tsqrt(double a) { return sqrt(a); }
+__device__ __forceinline__ float tsqrt(float a) { return sqrtf(a); }
+
+// real**integer by binary exponentiation (Fortran's real**integer).
+template <class R>
+__device__ __forceinline__ R ipow(R a, int b) {
+    bool neg = b < 0;
+    unsigned ub = neg ? 0u - (unsigned)b : (unsigned)b;
+    R r = R(1);
+    while (ub) {
+        if (ub & 1u) r *= a;
+        ub >>= 1;
+        if (ub) a *= a;
+    }
+    return neg ? R(1) / r : r;
+}
+
+// Cell state with primitives and fluxes (COMPUTE_PRIMITIVE_VARS outputs).
+template <class R>
+struct Prim {
+    R c[5];
+    R u, v, t, p;
+    R f[5], g[5];
+};
+
+// turbnstwod.f90:204-227
+template <class R>
+__device__ __forceinline__ void prim_from_q(Prim<R> &s, R q0, R q1, R q2, R q3, R q4,
+                                            const Consts<R> &K) {
+    R rho = q0;
+    s.c[0] = q0; s.c[1] = q1; s.c[2] = q2; s.c[3] = q3; s.c[4] = q4;
+    s.u = q1 / rho;
+    s.v = q2 / rho;
+    R e = q3 / rho;
+    s.t = (e - R(0.5) * (s.u * s.u + s.v * s.v)) / K.cv;
+    s.p = rho * K.rr * s.t;
+    s.f[0] = q1;
+    s.f[1] = rho * s.u * s.u + s.p;
+    s.f[2] = rho * s.u * s.v;
+    s.f[3] = (q3 + s.p) * s.u;
+    s.g[0] = q2;
+    s.g[1] = s.f[2];
+    s.g[2] = rho * s.v * s.v + s.p;
+    s.g[3] = (q3 + s.p) * s.v;
+    R rt = q4 / rho;
+    s.f[4] = rho * rt * s.u;
+    s.g[4] = rho * rt * s.v;
+}
+
+// u, v, T, p only (the first lines of turbnstwod.f90:97-114 / 204-227).
+template <class R>
+__device__ __forceinline__ void uvtp_from_q(R q0, R q1, R q2, R q3, R &u, R &v, R &t, R &p,
+                                            const Consts<R> &K) {
+    R rho = q0;
+    u = q1 / rho;
+    v = q2 / rho;
+    R e = q3 / rho;
+    t = (e - R(0.5) * (u * u + v * v)) / K.cv;
+    p = rho * K.rr * t;
+}
+
+// Common tail of boundary.f90:62-69, 89-93, 103-107, 321-325.
+template <class R>
+__device__ __forceinline__ void ghost_fill(Prim<R> &s, R rho, const Consts<R> &K) {
+    s.c[0] = rho;
+    s.c[1] = rho * s.u;
+    s.c[2] = rho * s.v;
+    R e = K.cv * s.t + R(0.5) * (s.u * s.u + s.v * s.v);
+    s.c[3] = rho * e;
+    s.f[0] = s.c[1];
+    s.f[1] = rho * s.u * s.u + s.p;
+    s.f[2] = rho * s.u * s.v;
+    s.f[3] = (s.c[3] + s.p) * s.u;
+    s.g[0] = s.c[2];
+    s.g[1] = s.f[2];
+    s.g[2] = rho * s.v * s.v + s.p;
+    s.g[3] = (s.c[3] + s.p) * s.v;
+}
+
+// Ghost-cell state for boundary types 2, 3, 4, 5, 7 (boundary.f90:53-122, 310-327).
+// par: parent state; (uo, vo, to, po): primitives of the parent's opposite
+// neighbour (type 3 only); ycc_g: ghost centroid y (type 7 only).
+template <class R>
+__device__ __forceinline__ void ghost_state(Prim<R> &s, int type, const Prim<R> &par, R uo, R vo,
+                                            R to, R po, R ycc_g, const Consts<R> &K) {
+    if (type == 2) {                       // subsonic isentropic inlet
+        s.u = fabs(par.u);
+        s.v = -par.v;
+        s.t = K.t0 - R(0.5) * (s.u * s.u + s.v * s.v) / K.cp;
+        R rho = tpow(K.s0 / (K.rr * s.t), K.inv_1mg);
+        s.p = rho * K.rr * s.t;
+        R amu = K.suth_sqrt_amol * tpow(s.t, R(0.6));
+        R rt = amu / rho;
+        ghost_fill(s, rho, K);
+        s.c[4] = rho * rt;
+        s.f[4] = s.c[4] * s.u;
+        s.g[4] = s.c[4] * s.v;
+    } else if (type == 3) {                // extrapolation "farfield"
+        s.u = fabs(R(2) * par.u - uo);
+        s.v = R(2) * par.v - vo;
+        s.t = R(2) * par.t - to;
+        s.p = R(2) * par.p - po;
+        if (K.pamb_set) s.p = K.pamb;
+        R rho = s.p / (K.rr * s.t);
+        ghost_fill(s, rho, K);
+        s.c[4] = par.c[4];
+        s.f[4] = par.f[4];
+        s.g[4] = par.g[4];
+    } else if (type == 4) {                // symmetry
+        s.u = par.u;
+        s.v = -par.v;
+        s.t = par.t;
+        s.p = par.p;
+        ghost_fill(s, par.c[0], K);
+        s.c[4] = par.c[4];
+        s.f[4] = par.f[4];
+        s.g[4] = -par.g[4];
+    } else if (type == 5) {                // no-slip wall / axis
+        s.u = -par.u;
+        s.v = -par.v;
+        s.t = par.t;
+        s.p = par.p;
+        R rho = par.c[0];
+        s.c[0] = rho;
+        s.c[1] = rho * s.u;
+        s.c[2] = rho * s.v;
+        R e = K.cv * s.t + R(0.5) * (s.u * s.u + s.v * s.v);
+        s.c[3] = rho * e;
+        s.f[0] = -par.f[0];
+        s.f[1] = R(2) * par.p - par.f[1];
+        s.f[2] = -par.f[2];
+        s.f[3] = -par.f[3];
+        s.g[0] = -par.g[0];
+        s.g[1] = -par.g[1];
+        s.g[2] = R(2) * par.p - par.g[2];
+        s.g[3] = -par.g[3];
+        s.c[4] = -par.c[4];
+        s.f[4] = -par.f[4];
+        s.g[4] = -par.g[4];
+    } else {                               // 7: 1/7-power boundary-layer inlet
+        s.p = K.pinf;
+        s.t = K.tinf;
+        s.v = -par.v;
+        R rho = s.p / (K.rr * s.t);
+        if (ycc_g <= K.deltabl)
+            s.u = K.bl_u * tpow(ycc_g / K.deltabl, R(kBlExponent)) - par.u;
+        else
+            s.u = K.bl_u - par.u;
+        R amu = K.suth_sqrt_amol * tpow(s.t, R(0.6));
+        R rt = amu / rho;
+        ghost_fill(s, rho, K);
+        s.c[4] = rho * rt;
+        s.f[4] = s.c[4] * s.u;
+        s.g[4] = s.c[4] * s.v;
+    }
+}
+
+// One face of VISCOUS_FACE (viscous.f90:130-217).  A = cell, B/D = face nodes,
+// C = neighbour.  Returns the face terms as pairs (a, b) so that the caller
+// accumulates VIS = (VIS + a) + b, the reference's order.
+template <class R>
+struct ViscFace {
+    R u0, u1, v0, v1, t0, t1, r0, r1;
+};
+
+template <class R>
+__device__ __forceinline__ ViscFace<R> viscous_face(R sx, R sy, R xa, R ya, R xb, R yb, R xc,
+                                                    R yc, R xd, R yd, R ua, R va, R ta, R rhoa,
+                                                    R ra, R ub, R vb, R tb, R rhob, R rb, R uc,
+                                                    R vc, R tc, R rhoc, R rc, R ud, R vd, R td,
+                                                    R rhod, R rd, const Consts<R> &K) {
+    R dnr = (xa - xc) * (yb - yd) - (xb - xd) * (ya - yc);
+    R dux = ((ua - uc) * (yb - yd) - (ub - ud) * (ya - yc)) / dnr;
+    R dvx = ((va - vc) * (yb - yd) - (vb - vd) * (ya - yc)) / dnr;
+    R dtx = ((ta - tc) * (yb - yd) - (tb - td) * (ya - yc)) / dnr;
+    R drx = ((ra - rc) * (yb - yd) - (rb - rd) * (ya - yc)) / dnr;
+    R duy = (-((ua - uc) * (xb - xd)) + (ub - ud) * (xa - xc)) / dnr;
+    R dvy = (-((va - vc) * (xb - xd)) + (vb - vd) * (xa - xc)) / dnr;
+    R dty = (-((ta - tc) * (xb - xd)) + (tb - td) * (xa - xc)) / dnr;
+    R dry = (-((ra - rc) * (xb - xd)) + (rb - rd) * (xa - xc)) / dnr;
+    R uavg = R(0.25) * (ua + ub + uc + ud);
+    R vavg = R(0.25) * (va + vb + vc + vd);
+    R tavg = R(0.25) * (ta + tb + tc + td);
+    R rhoavg = R(0.25) * (rhoa + rhob + rhoc + rhod);
+    R ravg = R(0.25) * (ra + rb + rc + rd);
+    R divvel;
+    if (K.iaxis == 1) {
+        R yavg = R(0.5) * (yb + yd);
+        R vbyy = (yavg < R(kYaxisThresh)) ? dvy : vavg / yavg;
+        divvel = dux + dvy + vbyy;
+    } else {
+        divvel = dux + dvy;
+    }
+    R amu = K.suth_sqrt_amol * tpow(tavg, R(0.6));
+    R ak = amu * K.cp / K.prandtl;
+    R chi = rhoavg * ravg / (amu * R(kCmu));
+    R fmu = ttanh(R(kAlphaTurb) * chi * chi) / ttanh(R(kBetaTurb) * chi * chi);
+    R amut = rhoavg * ravg * fmu;
+    R akt = amut * K.cp / K.prandtlt;
+    R amueff = amu + amut;
+    R akeff = ak + akt;
+    R txx = amueff * (dux + dux - R(kDiv23) * divvel);
+    R tyy = amueff * (dvy + dvy - R(kDiv23) * divvel);
+    R txy = amueff * (duy + dvx);
+    R qxx = uavg * txx + vavg * txy + akeff * dtx;
+    R qyy = uavg * txy + vavg * tyy + akeff * dty;
+    ViscFace<R> o;
+    o.u0 = txx * sx; o.u1 = txy * sy;
+    o.v0 = txy * sx; o.v1 = tyy * sy;
+    o.t0 = qxx * sx; o.t1 = qyy * sy;
+    o.r0 = amueff * drx * sx; o.r1 = amueff * dry * sy;
+    return o;
+}
+
+// COMPUTE_SOURCE_TERMS for one cell (source_terms.f90:40-101).  Neighbour
+// values a..d = across faces 1..4; r = C5/C1.  Returns SOURCE5, sets src3.
+template <class R>
+__device__ __forceinline__ R source_terms(R xa, R xb, R xc, R xd, R ya, R yb, R yc, R yd, R ua,
+                                          R ub, R uc, R ud, R va, R vb, R vc, R vd, R ra, R rb,
+                                          R rc, R rd, R rho, R c5, R t, R p, R v, R ycc,
+                                          R &src3, const Consts<R> &K) {
+    R qa = tsqrt(ua * ua + va * va), qb = tsqrt(ub * ub + vb * vb);
+    R qc = tsqrt(uc * uc + vc * vc), qd = tsqrt(ud * ud + vd * vd);
+    R dnr = (xa - xc) * (yb - yd) - (xb - xd) * (ya - yc);
+    R dqx = ((qa - qc) * (yb - yd) - (qb - qd) * (ya - yc)) / dnr;
+    R drx = ((ra - rc) * (yb - yd) - (rb - rd) * (ya - yc)) / dnr;
+    R dux = ((ua - uc) * (yb - yd) - (ub - ud) * (ya - yc)) / dnr;
+    R dvx = ((va - vc) * (yb - yd) - (vb - vd) * (ya - yc)) / dnr;
+    R dqy = (-((qa - qc) * (xb - xd)) + (qb - qd) * (xa - xc)) / dnr;
+    R dry = (-((ra - rc) * (xb - xd)) + (rb - rd) * (xa - xc)) / dnr;
+    R duy = (-((ua - uc) * (xb - xd)) + (ub - ud) * (xa - xc)) / dnr;
+    R dvy = (-((va - vc) * (xb - xd)) + (vb - vd) * (xa - xc)) / dnr;
+    R ri = c5 / rho;
+    R amu = K.suth_sqrt_amol * tpow(t, R(0.6));
+    R chi = rho * ri / (R(kCmu) * amu);
+    R fmu = ttanh(R(kAlphaTurb) * chi * chi) / ttanh(R(kBetaTurb) * chi * chi);
+    R fr2 = ttanh(tsqrt(chi)) / ttanh(chi * K.sqcmu);
+    R amut = fmu * rho * ri;
+    R phi = dqx * drx + dqy * dry;
+    R dee = R(0);
+    if (phi > R(0)) dee = drx * drx + dry * dry;
+    R ratio = ipow(amut / dmax(amut, R(kAmuThresh) * amu), K.npseudo);
+    R s5;
+    if (K.iaxis == 1) {
+        R vbyy = (ycc < R(kYaxisThresh)) ? dvy : v / ycc;
+        R divvel = dux + dvy + vbyy;
+        R pk = amut / rho *
+               (dux * dux + dvy * dvy + vbyy * vbyy + dux * dux + dvy * dvy + vbyy * vbyy +
+                (duy + dvx) * (duy + dvx) - R(kDiv23) * divvel * divvel);
+        pk = dmax(pk, R(0));
+        s5 = (rho * (R(kConst1) - R(kConst2) * fr2) * tsqrt(ri * pk) - rho * R(kConst3) * dee) *
+             ratio;
+        R ttt = (amu + amut) * (vbyy + vbyy - R(kDiv23) * divvel);
+        src3 = (p - ttt) / ycc;
+    } else {
+        R divvel = dux + dvy;
+        R pk = amut / rho *
+               (dux * dux + dvy * dvy + dux * dux + dvy * dvy + (duy + dvx) * (duy + dvx) -
+                R(kDiv23) * divvel * divvel);
+        pk = dmax(pk, R(0));
+        s5 = (rho * (R(kConst1) - R(kConst2) * fr2) * tsqrt(ri * pk) - rho * R(kConst3) * dee) *
+             ratio;
+        src3 = R(0);
+    }
+    return s5;
+}
+
+// Dissipation face coefficients (solver_state.f90:192-194): returns D2N, D4N.
+template <class R>
+__device__ __forceinline__ void jst_coeffs(R si, R sn, R volbydt, R vol_n, R dt_n, R &d2n,
+                                           R &d4n) {
+    R e2n = R(kC02) * dmax(si, sn);
+    R e4n = dmax(R(0), R(kC04) - e2n);
+    R d24n = R(0.5) * (volbydt + vol_n / dt_n);
+    d2n = e2n * d24n;
+    d4n = e4n * d24n;
+}
+
+}  // namespace tdt

@@ -1,0 +1,115 @@
+// Deterministic reductions: 64-lane wavefront shuffles, then a fixed-order
+// combination of the block's waves.  Sums are reproducible run to run because
+// the grid shape fixes the order; min / max are order-independent anyway.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace tdt {
+
+constexpr int kWave = 64;
+constexpr int kPartStride = 11;  // per block: 5 sums of squares, 5 maxima, 1 flag
+
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) x += __shfl_down(x, off, kWave);
+    return x;
+}
+
+__device__ __forceinline__ double wave_max(double x) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        double y = __shfl_down(x, off, kWave);
+        x = y > x ? y : x;
+    }
+    return x;
+}
+
+__device__ __forceinline__ double wave_min(double x) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        double y = __shfl_down(x, off, kWave);
+        x = y < x ? y : x;
+    }
+    return x;
+}
+
+// Block minimum of non-negative values folded into *slot with an atomic min on
+// the IEEE bit pattern (monotonic for non-negative doubles).
+__device__ __forceinline__ void block_atomic_min_pos(double x, double *slot) {
+    __shared__ double sh[16];
+    x = wave_min(x);
+    int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+    if (lane == 0) sh[w] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = sh[0];
+        int nw = (blockDim.x + kWave - 1) / kWave;
+        for (int q = 1; q < nw; ++q) m = sh[q] < m ? sh[q] : m;
+        if (m >= 0.0)
+            atomicMin(reinterpret_cast<unsigned long long *>(slot),
+                      (unsigned long long)__double_as_longlong(m));
+    }
+}
+
+// Writes this block's partial sums of squares, maxima and non-finite flag.
+__device__ __forceinline__ void block_partials_write(const double *sq, const double *mx,
+                                                     int bad, double *out) {
+    __shared__ double sh[16][kPartStride];
+    int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+    int nw = (blockDim.x + kWave - 1) / kWave;
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+        double s = wave_sum(sq[m]);
+        double x = wave_max(mx[m]);
+        if (lane == 0) { sh[w][m] = s; sh[w][5 + m] = x; }
+    }
+    double b = wave_max((double)bad);
+    if (lane == 0) sh[w][10] = b;
+    __syncthreads();
+    if (threadIdx.x < kPartStride) {
+        int q = threadIdx.x;
+        double acc = sh[0][q];
+        for (int k = 1; k < nw; ++k) {
+            double y = sh[k][q];
+            if (q < 5) acc = acc + y;
+            else acc = y > acc ? y : acc;
+        }
+        out[q] = acc;
+    }
+}
+
+// One block folds nblk partial records in a fixed order:
+// out[0..4] = sqrt(sum / n), out[5..9] = max, *flag |= any non-finite.
+__device__ __forceinline__ void reduce_partials(const double *part, int nblk, int n,
+                                                double *out, int *flag) {
+    __shared__ double sh[16][kPartStride];
+    double acc[kPartStride];
+    for (int q = 0; q < kPartStride; ++q) acc[q] = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
+        const double *p = part + (size_t)b * kPartStride;
+        for (int q = 0; q < 5; ++q) acc[q] = acc[q] + p[q];
+        for (int q = 5; q < kPartStride; ++q) acc[q] = p[q] > acc[q] ? p[q] : acc[q];
+    }
+    int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+    int nw = (blockDim.x + kWave - 1) / kWave;
+    for (int q = 0; q < kPartStride; ++q) {
+        double v = q < 5 ? wave_sum(acc[q]) : wave_max(acc[q]);
+        if (lane == 0) sh[w][q] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < kPartStride) {
+        int q = threadIdx.x;
+        double a = sh[0][q];
+        for (int k = 1; k < nw; ++k) {
+            double y = sh[k][q];
+            if (q < 5) a = a + y;
+            else a = y > a ? y : a;
+        }
+        if (q < 5) out[q] = sqrt(a / (double)n);
+        else if (q < 10) out[q] = a;
+        else if (a > 0.0) *flag = 1;
+    }
+}
+
+}  // namespace tdt

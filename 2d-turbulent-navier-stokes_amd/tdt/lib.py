@@ -1,0 +1,256 @@
+"""ctypes binding of libtdt (include/tdt.h).
+
+`Solver` mirrors the reference's routine boundary: one method per Fortran
+subroutine of the per-iteration path (timestep, dissipation, primitives,
+apply_boundary, viscous, source_terms, residual, update, monitor) plus the
+fused `iterate`.  All arrays are numpy, in the caller's original ordering.
+
+The library is the only compute path: if lib/libtdt.so is missing or no GPU
+is available, construction raises; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+from typing import Optional
+
+import numpy as np
+
+from .caseio import Flow, Mesh
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "lib", "libtdt.so")
+HEADER_PATH = os.path.join(os.path.dirname(PKG_DIR), "include", "tdt.h")
+
+TDT_OK, TDT_ERR_ARG, TDT_ERR_MESH, TDT_ERR_DEVICE, TDT_ERR_NONFINITE, TDT_ERR_UNSUPPORTED = (
+    0, -1, -2, -3, -4, -5)
+PRECISION = {"fp64": 0, "fp32-mixed": 1}
+ORDERING = {"input": 0, "rcm": 1, "auto": 2}
+ERROR_NAMES = {-1: "TDT_ERR_ARG", -2: "TDT_ERR_MESH", -3: "TDT_ERR_DEVICE",
+               -4: "TDT_ERR_NONFINITE", -5: "TDT_ERR_UNSUPPORTED"}
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int32)
+
+
+class TdtError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERROR_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+class _Mesh(ctypes.Structure):
+    _fields_ = [("nods", ctypes.c_int32), ("neles", ctypes.c_int32), ("nghosts", ctypes.c_int32),
+                ("x_mm", _dp), ("y_mm", _dp), ("nod", _ip), ("nc", _ip), ("parent", _ip),
+                ("ghost", _ip), ("type", _ip), ("side", _ip)]
+
+
+class _Params(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in (
+        "gama", "amol", "cfl", "prandtl", "prandtlt", "p0", "t0", "pamb", "amachinit", "rtred",
+        "pinf", "aminf", "deltabl")] + [(n, ctypes.c_int32) for n in (
+            "local", "npseudo", "iaxis", "reserved")]
+
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Loads libtdt.so and declares the prototypes of include/tdt.h."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FileNotFoundError(
+            f"{path} is missing: build it with `make -C {PKG_DIR}` (there is no CPU fallback)")
+    L = ctypes.CDLL(path)
+    vp = ctypes.c_void_p
+    L.tdt_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(_Mesh), ctypes.POINTER(_Params),
+                             ctypes.c_int32, ctypes.c_int32]
+    L.tdt_last_error.restype = ctypes.c_char_p
+    L.tdt_last_error.argtypes = [vp]
+    L.tdt_version.restype = ctypes.c_char_p
+    L.tdt_info.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
+    L.tdt_cold_start.argtypes = [vp, _dp]
+    L.tdt_set_state.argtypes = [vp, _dp, ctypes.c_double]
+    L.tdt_get_state.argtypes = [vp, _dp]
+    for fn in ("tdt_destroy", "tdt_timestep", "tdt_dissipation", "tdt_primitives",
+               "tdt_apply_boundary", "tdt_viscous", "tdt_source_terms", "tdt_residual",
+               "tdt_synchronize"):
+        getattr(L, fn).argtypes = [vp]
+    L.tdt_update.argtypes = [vp, ctypes.c_double]
+    L.tdt_monitor.argtypes = [vp, _dp, _dp]
+    L.tdt_iterate.argtypes = [vp, ctypes.c_int32, _dp, _dp, _ip]
+    L.tdt_get_array.argtypes = [vp, ctypes.c_char_p, _dp, ctypes.c_int32]
+    L.tdt_set_array.argtypes = [vp, ctypes.c_char_p, _dp, ctypes.c_int32]
+    L.tdt_set_profiling.argtypes = [vp, ctypes.c_int32]
+    L.tdt_kernel_times.argtypes = [vp, _dp]
+    _lib = L
+    return L
+
+
+def header_symbols(path: str = HEADER_PATH):
+    """Function names declared in include/tdt.h."""
+    text = open(path).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(tdt_[a-z0-9_]+)\s*\(", text)))
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+def make_params(flow: Flow) -> _Params:
+    return _Params(flow.gama, flow.amol, flow.cfl, flow.prandtl, flow.prandtlt, flow.p0, flow.t0,
+                   flow.pamb, flow.amachinit, flow.rtred, flow.pinf, flow.aminf, flow.deltabl,
+                   int(flow.local), int(flow.npseudo), int(flow.iaxis), 0)
+
+
+class Solver:
+    """One device-resident case: mesh, geometry, state Q and work arrays."""
+
+    CELL_GHOST = {"U", "V", "T", "P", "DT", "ANU", "ANURT", "XCC", "YCC", "VOL"} | \
+        {f"{p}{m}" for p in ("C", "F", "G", "D2Q") for m in range(1, 6)}
+    NODE = {"UN", "VN", "TN", "RHON", "RN"}
+    SCALAR = {"DTMIN", "RTOL", "RINIT"}
+
+    def __init__(self, mesh: Mesh, flow: Flow, precision: str = "fp64", ordering: str = "auto"):
+        self.L = load()
+        self.mesh, self.flow = mesh, flow
+        self._keep = [np.ascontiguousarray(a) for a in (
+            mesh.x_mm.astype(np.float64), mesh.y_mm.astype(np.float64), mesh.nod.astype(np.int32),
+            mesh.nc.astype(np.int32), mesh.parent.astype(np.int32), mesh.ghost.astype(np.int32),
+            mesh.btype.astype(np.int32), mesh.side.astype(np.int32))]
+        k = self._keep
+        m = _Mesh(mesh.nods, mesh.neles, mesh.nghosts, _d(k[0]), _d(k[1]), _i(k[2]), _i(k[3]),
+                  _i(k[4]), _i(k[5]), _i(k[6]), _i(k[7]))
+        p = make_params(flow)
+        h = ctypes.c_void_p()
+        rc = self.L.tdt_create(ctypes.byref(h), ctypes.byref(m), ctypes.byref(p),
+                               PRECISION[precision], ORDERING[ordering])
+        if rc != TDT_OK:
+            raise TdtError(rc, self.L.tdt_last_error(None).decode())
+        self.h = h
+        info = (ctypes.c_int64 * 8)()
+        self.L.tdt_info(h, info)
+        self.neles, self.nghosts, self.nods, self.nmax = info[0], info[1], info[2], info[3]
+        self.bandwidth, self.ordering_used = info[4], info[6]
+        self.precision = precision
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.tdt_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc):
+        if rc != TDT_OK:
+            raise TdtError(rc, self.L.tdt_last_error(self.h).decode())
+
+    # ---- state -----------------------------------------------------------------------
+    def cold_start(self) -> float:
+        r = ctypes.c_double(0.0)
+        self._check(self.L.tdt_cold_start(self.h, ctypes.byref(r)))
+        return r.value
+
+    def set_state(self, c: np.ndarray, rinit: float) -> None:
+        c = np.ascontiguousarray(c, dtype=np.float64).reshape(5, self.neles)
+        self._check(self.L.tdt_set_state(self.h, _d(c), float(rinit)))
+
+    def get_state(self) -> np.ndarray:
+        out = np.empty((5, self.neles))
+        self._check(self.L.tdt_get_state(self.h, _d(out)))
+        return out
+
+    # ---- routine-level entry points ---------------------------------------------------
+    def timestep(self): self._check(self.L.tdt_timestep(self.h))
+    def dissipation(self): self._check(self.L.tdt_dissipation(self.h))
+    def primitives(self): self._check(self.L.tdt_primitives(self.h))
+    def apply_boundary(self): self._check(self.L.tdt_apply_boundary(self.h))
+    def viscous(self): self._check(self.L.tdt_viscous(self.h))
+    def source_terms(self): self._check(self.L.tdt_source_terms(self.h))
+    def residual(self): self._check(self.L.tdt_residual(self.h))
+    def update(self, alpha: float): self._check(self.L.tdt_update(self.h, float(alpha)))
+
+    def monitor(self):
+        l2 = np.zeros(5)
+        linf = np.zeros(5)
+        self._check(self.L.tdt_monitor(self.h, _d(l2), _d(linf)))
+        return l2, linf
+
+    # ---- fused hot path ----------------------------------------------------------------
+    def iterate(self, niter: int, history: bool = True, check: bool = True):
+        """Returns (l2[niter,5], linf[niter,5], first_nonfinite)."""
+        l2 = np.zeros((niter, 5)) if history else None
+        linf = np.zeros((niter, 5)) if history else None
+        bad = ctypes.c_int32(0)
+        rc = self.L.tdt_iterate(self.h, int(niter), _d(l2) if history else None,
+                                _d(linf) if history else None, ctypes.byref(bad))
+        if rc == TDT_ERR_NONFINITE and not check:
+            return l2, linf, bad.value
+        self._check(rc)
+        return l2, linf, bad.value
+
+    # ---- arrays ----------------------------------------------------------------------
+    def array_len(self, name: str) -> int:
+        if name in self.SCALAR:
+            return 1
+        if name in self.CELL_GHOST:
+            return self.nmax
+        if name in self.NODE:
+            return self.nods
+        return self.neles
+
+    def get_array(self, name: str) -> np.ndarray:
+        n = self.array_len(name)
+        out = np.zeros(n)
+        self._check(self.L.tdt_get_array(self.h, name.encode(), _d(out), n))
+        return out
+
+    def set_array(self, name: str, values: np.ndarray) -> None:
+        n = self.array_len(name)
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        if v.shape != (n,):
+            raise ValueError(f"{name}: expected {n} values, got {v.shape}")
+        self._check(self.L.tdt_set_array(self.h, name.encode(), _d(v), n))
+
+    def set_profiling(self, on: bool) -> None:
+        self._check(self.L.tdt_set_profiling(self.h, 1 if on else 0))
+
+    def kernel_times(self) -> np.ndarray:
+        out = np.zeros(6)
+        self._check(self.L.tdt_kernel_times(self.h, _d(out)))
+        return out
+
+    def synchronize(self):
+        self._check(self.L.tdt_synchronize(self.h))
+
+
+def version() -> str:
+    return load().tdt_version().decode()
+
+
+def primitives_from_state(c: np.ndarray, flow: Flow) -> dict:
+    """rho, u, v, p, nu~ from C1..C5 with the reference's formulas (turbnstwod.f90:204-227)."""
+    rr = 8314.0 / flow.amol
+    cv = rr / (flow.gama - 1.0)
+    rho = c[0]
+    u = c[1] / rho
+    v = c[2] / rho
+    e = c[3] / rho
+    t = (e - 0.5 * (u * u + v * v)) / cv
+    p = rho * rr * t
+    return {"rho": rho, "u": u, "v": v, "p": p, "nut": c[4] / rho, "T": t}

@@ -1,0 +1,205 @@
+#!/usr/bin/env python3
+"""Throughput of the fused residual + RK4 path on one MI355X.
+
+A step is one full Jameson RK4 iteration (time step, JST dissipation, stage-1
+viscous fluxes, source terms, ghost cells, four stage updates, monitor) over
+the whole grid.  The workload is the 385 x 129 axisymmetric
+converging-diverging duct (49,665 cells), synthetic grid from the seed.
+
+Prints ONE JSON line: Mcell-iters/s of the FP64 path (the parity-checked
+path), the FP32-mixed path beside it with its error against FP64, the
+roofline of the dominant kernel and the reference CPU path timed on this
+host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "2d-turbulent-navier-stokes_amd"))
+
+import tdt  # noqa: E402
+from tdt import caseio, gridgen  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+KERNELS = ("k_pre", "k_dv", "k_stage1", "k_stage2", "k_stage3", "k_stage4")
+
+
+def alg_bytes_per_cell(precision: str, m_over_n: float, local: int) -> dict:
+    """Algorithmic bytes per interior cell for one launch of each fused kernel.
+
+    Every array a kernel reads or writes is counted once at its storage width;
+    neighbour and node re-reads are cache hits; ghost work (O(perimeter)) is
+    neglected.  s = width of stage / dissipation / viscous / node storage.
+    """
+    s = 8 if precision == "fp64" else 4
+    q, nc, nod, sc, vol, xy, dt, dl = 40, 16, 16, 64, 8, 16, 8, 8
+    pre_cells = q + dl + nc + (dt if local else 0) + 7 * s          # Q, DL, NC -> DT, ANU, ANURT, D2Q
+    pre_nodes = m_over_n * (4 + 16 + 16 + 5 * s)                    # CSR ptr, overrides, ~4 ids -> 5 avgs
+    dv = (q + dt + vol + 2 * s + 5 * s + nc + nod + sc + xy + m_over_n * (16 + 5 * s)
+          + 5 * s + 4 * s)                                          # -> D, VIS
+    common = nc + sc + vol + xy + dt + 5 * s + 4 * s                # geometry, DT, D, VIS
+    st1 = q + common + 5 * s
+    st23 = 5 * s + q + common + 5 * s
+    st4 = 5 * s + q + common + q
+    return {"k_pre": pre_cells + pre_nodes, "k_dv": dv, "k_stage1": st1, "k_stage2": st23,
+            "k_stage3": st23, "k_stage4": st4}
+
+
+def load_traffic(precision: str):
+    """Measured HBM bytes per launch of each kernel (rocprofv3 FETCH_SIZE + WRITE_SIZE,
+    scaled by a copy kernel of known size), if a committed summary exists."""
+    path = os.path.join(REPO, "profiles", "traffic_duct_385x129.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        return json.load(fh).get(precision)
+
+
+def run_gpu(mesh, flow, precision, steps, warmup):
+    with tdt.Solver(mesh, flow, precision) as s:
+        s.cold_start()
+        if warmup:
+            s.iterate(warmup, history=False)
+        s.synchronize()
+        t0 = time.perf_counter()
+        l2, _, bad = s.iterate(steps)      # returns after its stream has drained
+        s.synchronize()
+        elapsed = time.perf_counter() - t0
+        state = s.get_state()
+        # per-kernel device time: the same number of iterations launched eagerly
+        # with HIP events around each kernel on the library's stream
+        s.set_profiling(True)
+        s.iterate(steps, history=False)
+        ktimes = s.kernel_times()
+        s.set_profiling(False)
+    return elapsed, state, l2, bad, ktimes
+
+
+def cpu_baseline(mesh, flow, target_s: float = 15.0):
+    """The reference executable (built from its sources by oracle/build_ref.sh)
+    on this host, one core; loop time = t(NITER=k) - t(NITER=0)."""
+    ref = os.path.join(REPO, "oracle", "_ref", "turbnstwod")
+    kind = "reference"
+    if not os.path.exists(ref) or mesh.neles + mesh.nghosts > 60000:
+        return cpu_baseline_port(mesh, flow, target_s)
+    work = tempfile.mkdtemp(prefix="tdt_cpu_")
+    try:
+        def run(niter):
+            f = caseio.Flow(**{**flow.__dict__, "niter": niter, "istart": 0})
+            caseio.write_case(work, f, mesh)
+            t0 = time.perf_counter()
+            subprocess.run(["taskset", "-c", "0", ref] if shutil.which("taskset") else [ref],
+                           cwd=work, stdout=subprocess.DEVNULL, check=True)
+            return time.perf_counter() - t0
+        t_probe = run(5) - run(0)
+        per_iter = max(t_probe / 5, 1e-6)
+        k = int(max(10, min(2000, target_s / per_iter)))
+        t_k, t_0 = run(k), run(0)
+        loop = max(t_k - t_0, 1e-9)
+        value = mesh.neles * k / loop / 1e6
+        return {"value": round(value, 4), "unit": "Mcell-iters/s", "cores": 1, "kind": kind,
+                "sample": f"reference turbnstwod (FP64, amdflang -O3), same grid and flow.in, "
+                          f"{k} iterations, loop time = t(NITER={k}) - t(NITER=0) = {loop:.2f} s"}
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
+def cpu_baseline_port(mesh, flow, target_s):
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    from oracle import Oracle
+    o = Oracle(mesh, flow)
+    o.cold_start()
+    t0 = time.perf_counter()
+    o.iterate(2)
+    per = (time.perf_counter() - t0) / 2
+    k = int(max(2, min(2000, target_s / per)))
+    t0 = time.perf_counter()
+    o.iterate(k)
+    loop = time.perf_counter() - t0
+    return {"value": round(mesh.neles * k / loop / 1e6, 4), "unit": "Mcell-iters/s", "cores": 1,
+            "kind": "port", "sample": f"C oracle (FP64 restatement), {k} iterations, {loop:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--workload", default="duct_385x129", choices=sorted(gridgen.WORKLOADS))
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the FP32-mixed run")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); multi-GPU "
+              "sharding is a design, not an implementation", file=sys.stderr)
+        sys.exit(2)
+    mk_mesh, mk_flow = gridgen.WORKLOADS[args.workload]
+    mesh, flow = mk_mesh(), mk_flow()
+    n = mesh.neles
+    print(f"bench: {args.workload} neles={n} nghosts={mesh.nghosts} steps={args.steps} "
+          f"warmup={args.warmup}", file=sys.stderr, flush=True)
+
+    el64, st64, l2_64, bad64, kt64 = run_gpu(mesh, flow, "fp64", args.steps, args.warmup)
+    v64 = n * args.steps / el64 / 1e6
+    print(f"bench: fp64 {v64:.1f} Mcell-iters/s ({el64 * 1e3 / args.steps:.4f} ms/iter) "
+          f"kernels(ms) {np.round(kt64, 5).tolist()}", file=sys.stderr, flush=True)
+    mixed = None
+    if not args.no_mixed:
+        el32, st32, _, bad32, kt32 = run_gpu(mesh, flow, "fp32-mixed", args.steps, args.warmup)
+        v32 = n * args.steps / el32 / 1e6
+        a = tdt.primitives_from_state(st32, flow)
+        b = tdt.primitives_from_state(st64, flow)
+        err = {k: float(np.linalg.norm(a[k] - b[k]) / max(np.linalg.norm(b[k]), 1e-300))
+               for k in ("p", "rho", "u", "nut")}
+        mixed = {"value": round(v32, 2), "ms_per_step": round(el32 * 1e3 / args.steps, 5),
+                 "speedup_vs_fp64": round(v32 / v64, 3),
+                 "rel_l2_vs_fp64_after_steps": {k: float(f"{v:.3e}") for k, v in err.items()},
+                 "kernel_ms": [round(float(x), 5) for x in kt32], "nonfinite_iter": int(bad32)}
+        print(f"bench: fp32-mixed {v32:.1f} Mcell-iters/s, rel err vs fp64 {err}", file=sys.stderr,
+              flush=True)
+
+    # roofline of the dominant kernel (largest average duration per launch)
+    b = alg_bytes_per_cell("fp64", mesh.nods / n, flow.local)
+    dom = int(np.argmax(kt64))
+    name = KERNELS[dom]
+    t_launch = kt64[dom] * 1e-3
+    achieved = b[name] * n / t_launch / 1e9
+    traffic = load_traffic("fp64")
+    roof = {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": (traffic or {}).get(name),
+            "alg_bytes_per_cell": round(b[name], 1), "cells_per_launch": n,
+            "avg_launch_ms": round(kt64[dom], 5),
+            "alg_bytes_per_cell_iter": round(sum(b.values()), 1),
+            "iteration_alg_GBs": round(sum(b.values()) * n * args.steps / el64 / 1e9, 1)}
+
+    cpu = None if args.no_cpu else cpu_baseline(mesh, flow)
+    line = {
+        "metric": "Mcell-iters/s on 1 MI355X, FP64 vs FP32-mixed at fixed error; % HBM peak",
+        "value": round(v64, 2), "unit": "Mcell-iters/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(el64 * 1e3 / args.steps, 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded grid generator, cold start from uniform flow)",
+        "config": {"workload": f"{args.workload}: axisymmetric converging-diverging duct, "
+                               f"{n} cells + {mesh.nghosts} ghosts, IAXIS={flow.iaxis}, "
+                               f"P0={flow.p0}, T0={flow.t0}, M={flow.amachinit}, CFL={flow.cfl}",
+                   "precision": "fp64", "nonfinite_iter": int(bad64)},
+        "fp32_mixed": mixed,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

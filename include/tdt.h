@@ -1,0 +1,135 @@
+/*
+ * tdt.h -- C ABI of libtdt, the MI355X (gfx950) residual + Jameson RK4 engine
+ * for the 2-D / axisymmetric turbulent Navier-Stokes solver.
+ *
+ * The reference has no library interface: its hot path is a set of
+ * argument-less Fortran subroutines sharing module state.  Each entry point
+ * below replaces one of them (file:line of the reference routine given), so a
+ * Fortran host can keep its CALL sites and bind these through iso_c_binding.
+ *
+ * Conventions
+ *   - every function returns TDT_OK (0) or a negative TDT_ERR_* code;
+ *     tdt_last_error(ctx) gives the message (ctx may be NULL for failures of
+ *     tdt_create);
+ *   - ids are 1-based as in grid.in / bc.in; arrays are host memory, FP64 or
+ *     int32, contiguous; per-cell arrays with several components are
+ *     column-major [component][neles] (Fortran C1..C5(NELES) / NOD(NELES,4));
+ *   - the library renumbers cells and nodes internally; every array crossing
+ *     this ABI is in the caller's (original) order;
+ *   - the conservative state Q = [rho, rho u, rho v, rho E, rho nu~] stays in
+ *     device memory between tdt_set_state / tdt_cold_start and
+ *     tdt_get_state;
+ *   - a context is used by one host thread at a time; contexts are
+ *     independent.
+ */
+#ifndef TDT_H
+#define TDT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TDT_OK 0
+#define TDT_ERR_ARG (-1)         /* bad argument or array length            */
+#define TDT_ERR_MESH (-2)        /* inconsistent grid.in / bc.in data       */
+#define TDT_ERR_DEVICE (-3)      /* HIP runtime error or no usable GPU      */
+#define TDT_ERR_NONFINITE (-4)   /* NaN/Inf appeared in the state           */
+#define TDT_ERR_UNSUPPORTED (-5) /* feature outside this library's scope    */
+
+#define TDT_FP64 0        /* FP64 everywhere (parity path)                          */
+#define TDT_FP32_MIXED 1  /* FP32 stage storage + flux/dissipation/viscous/source
+                             arithmetic; FP64 geometry, Q^n, update and norms       */
+
+#define TDT_ORDER_INPUT 0 /* keep grid.in cell order                       */
+#define TDT_ORDER_RCM 1   /* reverse Cuthill-McKee                         */
+#define TDT_ORDER_AUTO 2  /* whichever of the two has the smaller bandwidth */
+
+typedef struct tdt_ctx tdt_ctx;
+
+/* Grid and boundary lists as read by COMPUTE_GEOMETRY (geometry.f90:31-52). */
+typedef struct tdt_mesh {
+    int32_t nods, neles, nghosts;
+    const double *x_mm, *y_mm;   /* [nods] node coordinates in mm (geometry.f90:34-36) */
+    const int32_t *nod;          /* [4][neles] NOD(I,K), counter-clockwise              */
+    const int32_t *nc;           /* [4][neles] NCk(I): cell id, ghost id, or -1 (then
+                                    resolved through bc.in's PARENT/SIDE)               */
+    const int32_t *parent, *ghost, *type, *side; /* [nghosts] bc.in columns          */
+} tdt_mesh;
+
+/* flow.in fields (turbnstwod.f90:20-23) and the BC type-7 inputs PINF, AMINF,
+ * DELTABL that the reference reads but never assigns (boundary.f90:312-317). */
+typedef struct tdt_params {
+    double gama, amol, cfl, prandtl, prandtlt, p0, t0, pamb, amachinit, rtred;
+    double pinf, aminf, deltabl;
+    int32_t local, npseudo, iaxis, reserved;
+} tdt_params;
+
+/* ---- lifecycle ---------------------------------------------------------------- */
+/* Validates and renumbers the mesh, uploads it, computes the geometry on the GPU
+ * (geometry.f90:55-149) and the thermodynamic constants (turbnstwod.f90:37-42).
+ * precision: TDT_FP64 or TDT_FP32_MIXED; ordering: TDT_ORDER_*.            */
+int tdt_create(tdt_ctx **ctx, const tdt_mesh *mesh, const tdt_params *params,
+               int32_t precision, int32_t ordering);
+int tdt_destroy(tdt_ctx *ctx);
+const char *tdt_last_error(const tdt_ctx *ctx);
+const char *tdt_version(void);
+/* info[0..7] = neles, nghosts, nods, nmax (largest cell/ghost id), bandwidth of the
+ * internal order, precision, ordering actually used, device index */
+int tdt_info(const tdt_ctx *ctx, int64_t *info);
+
+/* ---- state -------------------------------------------------------------------- */
+/* Uniform start state; returns RINIT (turbnstwod.f90:53-71); COLD <- C. */
+int tdt_cold_start(tdt_ctx *ctx, double *rinit);
+/* Restart state c = [5][neles] (C1..C5) and RINIT (turbnstwod.f90:73-90); COLD <- C. */
+int tdt_set_state(tdt_ctx *ctx, const double *c, double rinit);
+int tdt_get_state(tdt_ctx *ctx, double *c);
+
+/* ---- routine-level entry points (FP64; same semantics as the reference) -------- */
+int tdt_timestep(tdt_ctx *ctx);            /* turbnstwod.f90:97-114               */
+int tdt_dissipation(tdt_ctx *ctx);         /* DISSIPATION   solver_state.f90:70   */
+int tdt_primitives(tdt_ctx *ctx);          /* COMPUTE_PRIMITIVE_VARS turbnstwod.f90:204 */
+int tdt_apply_boundary(tdt_ctx *ctx);      /* APPLY_BOUNDARY boundary.f90:24      */
+int tdt_viscous(tdt_ctx *ctx);             /* COMPUTE_VISCOUS viscous.f90:29      */
+int tdt_source_terms(tdt_ctx *ctx);        /* COMPUTE_SOURCE_TERMS source_terms.f90:24 */
+int tdt_residual(tdt_ctx *ctx);            /* COMPUTE_RESIDUAL residual.f90:23    */
+int tdt_update(tdt_ctx *ctx, double alpha); /* UPDATE_SOLUTION(ALPHA) residual.f90:67 */
+/* Per-equation L2 = sqrt(sum (C-COLD)^2 / NELES) and Linf = max |C-COLD|, then
+ * COLD <- C (turbnstwod.f90:147-168).  l2, linf: [5] or NULL.  Returns
+ * TDT_ERR_NONFINITE if the state holds NaN/Inf (the reference masks it). */
+int tdt_monitor(tdt_ctx *ctx, double *l2, double *linf);
+
+/* ---- fused hot path ------------------------------------------------------------- */
+/* niter full RK4 iterations (turbnstwod.f90:95-168) on the device: one pre-pass
+ * (time step, dissipation sensors, nodal averages) and one fused kernel per RK
+ * stage, launched as a HIP graph.  l2, linf: [niter][5] or NULL.
+ * first_nonfinite (may be NULL) receives the first iteration (1-based) whose
+ * state held NaN/Inf, or 0; the call then returns TDT_ERR_NONFINITE.
+ * Leaves C and COLD equal, as at the end of a reference iteration. */
+int tdt_iterate(tdt_ctx *ctx, int32_t niter, double *l2, double *linf,
+                int32_t *first_nonfinite);
+
+/* ---- arrays (reference names, original ids) --------------------------------------
+ * Cell+ghost arrays have nmax entries (index = id-1; ids that are neither cell nor
+ * ghost read 0): C1..C5 U V T P DT F1..F5 G1..G5 D2Q1..D2Q5 ANU ANURT XCC YCC VOL.
+ * Per-cell arrays have neles entries: COLD1..5 R1..R5 D241..D245 VISU VISV VIST
+ * VISR SOURCE3 SOURCE5 DL SC1X SC1Y .. SC4X SC4Y.  Node arrays have nods entries:
+ * UN VN TN RHON RN.  Scalars (len 1): DTMIN RTOL RINIT.                           */
+int tdt_get_array(tdt_ctx *ctx, const char *name, double *out, int32_t len);
+int tdt_set_array(tdt_ctx *ctx, const char *name, const double *in, int32_t len);
+
+/* ---- measurement -------------------------------------------------------------- */
+/* Average device time (ms) per launch of each fused kernel over the last
+ * tdt_iterate call, measured with HIP events on the library's stream when
+ * enabled (kernels are then launched eagerly instead of as a graph; 0
+ * disables, the default).  times: [6] = pre-pass, dissipation+viscous,
+ * stage 1, stage 2, stage 3, stage 4. */
+int tdt_set_profiling(tdt_ctx *ctx, int32_t enable);
+int tdt_kernel_times(tdt_ctx *ctx, double *times_ms);
+int tdt_synchronize(tdt_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TDT_H */

@@ -1,0 +1,160 @@
+"""Fused hot path (tdt_iterate) on the GPU against the reference and the routine path.
+
+Bar (the FP64 parity target): the per-iteration L2 history of C - COLD and the
+final rho, u, v, p, nu~ fields agree with the reference to 1e-10 relative; the
+only permitted differences are reduction order (the L2 sums) and libm
+(pow / tanh) rounding.
+"""
+import numpy as np
+import pytest
+
+from conftest import DUMP_CASES, RUN_CASES
+
+pytestmark = pytest.mark.gpu
+
+ALPHAS = (0.25, 1.0 / 3.0, 0.5, 1.0)
+REL = 1e-10
+
+
+def reference_run(g):
+    """History and final state of the fixture (dump cases: nrun + the dump iteration)."""
+    l2 = g["hist_l2"].reshape(-1, 5)
+    linf = g["hist_linf"].reshape(-1, 5)
+    n = g.nrun
+    if g.has("d_L2"):
+        l2 = np.vstack([l2, g["d_L2"][None, :]])
+        linf = np.vstack([linf, g["d_LINF"][None, :]])
+        n += 1
+    return n, l2, linf
+
+
+def field_errors(got, want, flow):
+    import tdt
+    a = tdt.primitives_from_state(got, flow)
+    b = tdt.primitives_from_state(want, flow)
+    out = {}
+    for k in ("rho", "u", "v", "p", "nut"):
+        den = np.linalg.norm(b[k])
+        out[k] = np.linalg.norm(a[k] - b[k]) / (den if den > 0 else 1.0)
+    return out
+
+
+@pytest.mark.parametrize("case", DUMP_CASES + RUN_CASES)
+def test_fused_matches_reference(golden, case):
+    import tdt
+    g = golden(case)
+    n, l2_ref, linf_ref = reference_run(g)
+    with tdt.Solver(g.mesh, g.flow, "fp64") as s:
+        s.cold_start()
+        l2, linf, bad = s.iterate(n)
+        assert bad == 0
+        final = s.get_state()
+    # per-iteration history
+    scale = np.maximum(np.abs(l2_ref), 1e-300)
+    err_l2 = np.max(np.abs(l2 - l2_ref) / scale)
+    assert err_l2 <= REL, f"L2 history: max relative deviation {err_l2:.3e}"
+    err_linf = np.max(np.abs(linf - linf_ref) / np.maximum(np.abs(linf_ref), 1e-300))
+    assert err_linf <= REL, f"Linf history: max relative deviation {err_linf:.3e}"
+    if g.has("final_idx"):
+        final = final[:, g["final_idx"]]
+    errs = field_errors(final, g["final"], g.flow)
+    worst = max(errs.values())
+    print(f"{case}: {n} iterations, L2-history rel dev {err_l2:.2e}, Linf {err_linf:.2e}, "
+          f"fields {({k: float('%.2e' % v) for k, v in errs.items()})}")
+    assert worst <= REL, f"final fields relative L2 errors {errs}"
+
+
+@pytest.mark.parametrize("case", ["t2d_chan", "t2d_ramp", "taxi_duct", "t7_plate", "case1_axi"])
+def test_fused_equals_routine_path(golden, case):
+    """The fused kernels and the routine-level kernels compute the same
+    expressions in the same order: the states must be bitwise identical."""
+    import tdt
+    g = golden(case)
+    k = 7
+    with tdt.Solver(g.mesh, g.flow, "fp64") as a, tdt.Solver(g.mesh, g.flow, "fp64") as b:
+        a.cold_start()
+        b.cold_start()
+        for _ in range(k):
+            a.timestep()
+            a.dissipation()
+            for st in range(4):
+                a.primitives()
+                a.apply_boundary()
+                if st == 0:
+                    a.viscous()
+                a.source_terms()
+                a.residual()
+                a.update(ALPHAS[st])
+            a.monitor()
+        b.iterate(k)
+        sa, sb = a.get_state(), b.get_state()
+        assert np.array_equal(sa, sb), f"max diff {np.max(np.abs(sa - sb))}"
+        ne = g.mesh.neles
+        for nm in ("D241", "D245", "VISU", "VIST", "VISR", "UN", "RN", "DT"):
+            assert np.array_equal(a.get_array(nm)[:ne], b.get_array(nm)[:ne]), nm
+        # sensors: the routine path also keeps the reference's ghost copies
+        for nm in ("ANU", "ANURT", "D2Q1", "D2Q5"):
+            assert np.array_equal(a.get_array(nm)[:ne], b.get_array(nm)[:ne]), nm
+
+
+@pytest.mark.parametrize("case", ["case1", "taxi_duct", "t2d_ramp"])
+def test_ordering_invariance_and_determinism(golden, case):
+    """Renumbering changes memory layout only: input order and RCM order give
+    bitwise identical states, and repeated runs are identical."""
+    import tdt
+    g = golden(case)
+    states = []
+    for ordering in ("input", "rcm", "rcm"):
+        with tdt.Solver(g.mesh, g.flow, "fp64", ordering) as s:
+            s.cold_start()
+            l2, _, _ = s.iterate(12)
+            states.append((s.get_state(), l2))
+    assert np.array_equal(states[0][0], states[1][0])
+    assert np.array_equal(states[1][0], states[2][0])
+    assert np.array_equal(states[1][1], states[2][1])
+
+
+def test_restart_is_exact(golden):
+    """Stopping, reading the state back and restarting continues bit for bit
+    (COLD is recomputed exactly because COLD = C at iteration end)."""
+    import tdt
+    g = golden("case1")
+    with tdt.Solver(g.mesh, g.flow, "fp64") as a, tdt.Solver(g.mesh, g.flow, "fp64") as b:
+        rinit = a.cold_start()
+        a.iterate(10)
+        mid = a.get_state()
+        a.iterate(10)
+        b.set_state(mid, rinit)
+        b.iterate(10)
+        assert np.array_equal(a.get_state(), b.get_state())
+
+
+def test_nonfinite_is_reported(golden):
+    """A NaN in the state is reported with the iteration where it appeared,
+    instead of being masked as the reference's monitor does."""
+    import tdt
+    g = golden("t2d_chan")
+    with tdt.Solver(g.mesh, g.flow, "fp64") as s:
+        rinit = s.cold_start()
+        c = s.get_state()
+        c[3, 5] = np.nan
+        s.set_state(c, rinit)
+        _, _, bad = s.iterate(3, check=False)
+        assert bad == 1
+        with pytest.raises(tdt.TdtError) as ei:
+            s.iterate(1)
+        assert ei.value.code == -4
+
+
+def test_fp32_mixed_tracks_fp64(golden):
+    """FP32-mixed storage/arithmetic stays close to FP64 over a short run."""
+    import tdt
+    g = golden("case1")
+    with tdt.Solver(g.mesh, g.flow, "fp64") as a, tdt.Solver(g.mesh, g.flow, "fp32-mixed") as b:
+        a.cold_start()
+        b.cold_start()
+        a.iterate(50)
+        b.iterate(50)
+        errs = field_errors(b.get_state(), a.get_state(), g.flow)
+    assert errs["p"] < 1e-5, errs
+    assert errs["rho"] < 1e-5, errs

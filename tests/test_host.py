@@ -1,0 +1,115 @@
+"""Host-side tests that need no GPU: the C ABI library, case I/O, grid
+generation and mesh validation."""
+import dataclasses
+import os
+
+import numpy as np
+import pytest
+
+import tdt
+from tdt import caseio, gridgen, lib
+
+
+def test_library_exports_every_declared_symbol():
+    L = tdt.load()
+    syms = lib.header_symbols()
+    assert len(syms) >= 20
+    missing = [s for s in syms if not hasattr(L, s)]
+    assert not missing, missing
+    assert tdt.version().startswith("tdt")
+
+
+def test_case_files_round_trip(tmp_path, golden):
+    g = golden("t2d_ramp")
+    caseio.write_case(str(tmp_path), g.flow, g.mesh)
+    flow, mesh = caseio.load_case(str(tmp_path))
+    assert dataclasses.replace(flow, pinf=g.flow.pinf, aminf=g.flow.aminf,
+                               deltabl=g.flow.deltabl) == g.flow
+    for k, v in g.mesh.arrays().items():
+        assert np.array_equal(getattr(mesh, k), v), k
+    c = g["final"]
+    caseio.write_restart(str(tmp_path / "restart.in"), c, 1e-5, 1, 1.2345678901234567e-5, 30)
+    c2, rtred, npseudo, rinit, niterold = caseio.read_restart(str(tmp_path / "restart.in"),
+                                                              g.mesh.neles)
+    assert np.array_equal(c, c2)
+    assert (rtred, npseudo, rinit, niterold) == (1e-5, 1, 1.2345678901234567e-5, 30)
+
+
+def test_truncated_restart_is_rejected(tmp_path, golden):
+    g = golden("t2d_chan")
+    caseio.write_restart(str(tmp_path / "r.in"), g["final"], 1e-5, 1, 1e-5, 30)
+    text = open(tmp_path / "r.in").read().split("\n")
+    open(tmp_path / "r.in", "w").write("\n".join(text[:10]))
+    with pytest.raises(ValueError):
+        caseio.read_restart(str(tmp_path / "r.in"), g.mesh.neles)
+
+
+def test_duct_generator_reproduces_fixture(golden):
+    g = golden("duct_385x129")
+    m = gridgen.duct()
+    for k, v in g.mesh.arrays().items():
+        assert np.array_equal(getattr(m, k), v), k
+
+
+def test_structured_numbering_matches_example_case_convention():
+    m = gridgen.channel(nx=4, ny=3, bc={"left": 2, "right": 3, "bottom": 4, "top": 5})
+    # cell 1: nodes (0,0),(1,0),(1,1),(0,1) -> 1, ny+2, ny+3, 2
+    assert list(m.nod[:, 0]) == [1, 5, 6, 2]
+    # NC2 of cell 1 is the cell to the right (id ny+1), NC3 the one above (id 2)
+    assert m.nc[1, 0] == 4 and m.nc[2, 0] == 2
+    # first ghosts: cell 1 bottom (side 1, type 4), then cell 1 left (side 4, type 2)
+    assert list(zip(m.parent[:2], m.ghost[:2], m.btype[:2], m.side[:2])) == [
+        (1, 13, 4, 1), (1, 14, 2, 4)]
+    assert m.nghosts == 2 * (4 + 3)
+
+
+def test_geometric_stretching():
+    q = gridgen.geometric_ratio(0.01, 50.0, 49)
+    assert abs(q - 1.143683) < 1e-6
+    s = gridgen.stretched(0.01, 50.0, 49)
+    assert s[0] == 0.0 and s[-1] == 50.0 and abs(s[1] - 0.01) < 1e-12
+
+
+def _create(mesh, flow):
+    return tdt.Solver(mesh, flow)
+
+
+@pytest.mark.parametrize("mutate,code", [
+    (lambda m: m.ghost.__setitem__(0, 3), lib.TDT_ERR_MESH),          # collides with a cell id
+    (lambda m: m.btype.__setitem__(0, 6), lib.TDT_ERR_UNSUPPORTED),   # combustion boundary
+    (lambda m: m.btype.__setitem__(0, 9), lib.TDT_ERR_MESH),          # unknown type
+    (lambda m: m.side.__setitem__(0, 3), lib.TDT_ERR_MESH),           # wrong face
+    (lambda m: m.nc.__setitem__((1, 0), 10**6), lib.TDT_ERR_MESH),    # dangling neighbour
+    (lambda m: m.nod.__setitem__((0, 0), 0), lib.TDT_ERR_MESH),       # node id out of range
+])
+def test_mesh_validation(golden, mutate, code):
+    g = golden("t2d_chan")
+    m = caseio.Mesh.from_arrays({k: v.copy() for k, v in g.mesh.arrays().items()})
+    mutate(m)
+    with pytest.raises(tdt.TdtError) as ei:
+        _create(m, g.flow)
+    assert ei.value.code == code, str(ei.value)
+
+
+def test_minus_one_neighbours_resolve_through_bc(golden):
+    """NCk = -1 on a boundary face is accepted and resolved through bc.in
+    (PARENT, SIDE); validation then succeeds and only the device check can fail."""
+    g = golden("t2d_chan")
+    m = caseio.Mesh.from_arrays({k: v.copy() for k, v in g.mesh.arrays().items()})
+    m.nc[m.nc > m.neles] = -1
+    try:
+        s = _create(m, g.flow)
+        s.close()
+    except tdt.TdtError as e:
+        assert e.code == lib.TDT_ERR_DEVICE, str(e)
+
+
+def test_no_gpu_fails_loudly(golden):
+    """Without a GPU the library refuses to run; there is no CPU fallback."""
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    g = golden("t2d_chan")
+    with pytest.raises(tdt.TdtError) as ei:
+        _create(g.mesh, g.flow)
+    assert ei.value.code == lib.TDT_ERR_DEVICE
