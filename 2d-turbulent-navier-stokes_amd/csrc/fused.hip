@@ -14,13 +14,17 @@
 //   k_stage<s>   s = 1..4: primitives, ghost cells (boundary.f90) computed
 //                inline from the parent, source terms, central flux residual
 //                and the RK update (residual.f90, source_terms.f90).  Stage 4
-//                also folds the L2 / Linf monitor into a last-block reduction.
+//                also writes per-block L2 / Linf partials of C - COLD, which
+//                block 0 of the next pre-pass (or k_finalize) folds in a fixed
+//                order into the per-iteration history.
 // Every kernel is a cell- or node-centric gather: no atomics on values, so
 // results are deterministic.  Ghost neighbours of a cell are only ever
 // consumed by that cell (its parent), which is what lets the stage kernels
 // compute them on the fly instead of running a separate boundary pass.
 #include "ctx.hpp"
 #include "reduce.hpp"
+
+#include <hip/hip_ext.h>
 
 namespace tdt {
 
@@ -403,29 +407,27 @@ __global__ void __launch_bounds__(kBlock) k_cold_from_qn(FDev F, double *cold) {
         for (int q = 0; q < 5; ++q) cold[q * F.n + i] = F.qn[q * F.nt + i];
 }
 
+// Launches one kernel; when profiling, with start/stop events that the runtime
+// stamps at the kernel's own begin and end (same durations a kernel trace shows).
+template <class K, class... Args>
+void launch(tdt_ctx *c, FusedBufs &B, int slot, K kern, unsigned grid, Args... args) {
+    if (c->profiling && B.have_events)
+        hipExtLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, c->stream, B.ev[2 * slot],
+                              B.ev[2 * slot + 1], 0, args...);
+    else
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, c->stream, args...);
+}
+
 template <class R, class SS, class SD>
 hipError_t enqueue_iteration(tdt_ctx *c, FusedBufs &B, const Consts<R> &KR) {
     const FDev &F = B.F;
-    hipStream_t s = c->stream;
-    bool ev = c->profiling && B.have_events;
-    if (ev) (void)hipEventRecord(B.ev[0], s);
-    hipLaunchKernelGGL((k_pre<SD, R>), dim3(B.nb_cells + B.nb_nodes), dim3(kBlock), 0, s, F, c->K,
-                       KR, B.nb_cells);
-    if (ev) (void)hipEventRecord(B.ev[1], s);
-    hipLaunchKernelGGL((k_dv<SD, R>), dim3(B.nb_cells), dim3(kBlock), 0, s, F, c->K, KR);
-    if (ev) (void)hipEventRecord(B.ev[2], s);
-    hipLaunchKernelGGL((k_stage<R, double, SS, SD, 1>), dim3(B.nb_cells), dim3(kBlock), 0, s, F,
-                       c->K, KR, 0.25);
-    if (ev) (void)hipEventRecord(B.ev[3], s);
-    hipLaunchKernelGGL((k_stage<R, SS, SS, SD, 2>), dim3(B.nb_cells), dim3(kBlock), 0, s, F, c->K,
-                       KR, 1.0 / 3.0);
-    if (ev) (void)hipEventRecord(B.ev[4], s);
-    hipLaunchKernelGGL((k_stage<R, SS, SS, SD, 3>), dim3(B.nb_cells), dim3(kBlock), 0, s, F, c->K,
-                       KR, 0.5);
-    if (ev) (void)hipEventRecord(B.ev[5], s);
-    hipLaunchKernelGGL((k_stage<R, SS, double, SD, 4>), dim3(B.nb_cells), dim3(kBlock), 0, s, F,
-                       c->K, KR, 1.0);
-    if (ev) (void)hipEventRecord(B.ev[6], s);
+    const unsigned nbc = B.nb_cells;
+    launch(c, B, 0, k_pre<SD, R>, nbc + B.nb_nodes, F, c->K, KR, B.nb_cells);
+    launch(c, B, 1, k_dv<SD, R>, nbc, F, c->K, KR);
+    launch(c, B, 2, k_stage<R, double, SS, SD, 1>, nbc, F, c->K, KR, 0.25);
+    launch(c, B, 3, k_stage<R, SS, SS, SD, 2>, nbc, F, c->K, KR, 1.0 / 3.0);
+    launch(c, B, 4, k_stage<R, SS, SS, SD, 3>, nbc, F, c->K, KR, 0.5);
+    launch(c, B, 5, k_stage<R, SS, double, SD, 4>, nbc, F, c->K, KR, 1.0);
     return hipGetLastError();
 }
 
@@ -551,10 +553,10 @@ int fused_iterate(tdt_ctx *c, int niter, double *l2, double *linf, int *first_no
         // eager launches bracketed by events: per-kernel device time
         for (int it = 0; it < niter && e == hipSuccess; ++it) {
             e = enqueue(c, B);
-            if (e == hipSuccess) e = hipEventSynchronize(B.ev[kNumFusedKernels]);
+            if (e == hipSuccess) e = hipEventSynchronize(B.ev[2 * kNumFusedKernels - 1]);
             for (int q = 0; q < kNumFusedKernels && e == hipSuccess; ++q) {
                 float ms = 0.f;
-                e = hipEventElapsedTime(&ms, B.ev[q], B.ev[q + 1]);
+                e = hipEventElapsedTime(&ms, B.ev[2 * q], B.ev[2 * q + 1]);
                 c->ktimes[q] += ms;
             }
         }

@@ -130,6 +130,19 @@ def cpu_baseline_port(mesh, flow, target_s):
             "kind": "port", "sample": f"C oracle (FP64 restatement), {k} iterations, {loop:.2f} s"}
 
 
+def copy_probe(mib: int) -> None:
+    """A plain elementwise copy of known size: y = x * 1 over MIB MiB of float64, three
+    times.  Its FETCH_SIZE / WRITE_SIZE calibrate the HBM counters (expect MIB MiB each)."""
+    import torch
+    n = mib * 1024 * 1024 // 8
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        torch.mul(x, 1.0, out=y)
+    torch.cuda.synchronize()
+    print(f"copy probe: {mib} MiB read + {mib} MiB written per launch, 3 launches", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -138,7 +151,21 @@ def main():
     ap.add_argument("--workload", default="duct_385x129", choices=sorted(gridgen.WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-mixed", action="store_true", help="skip the FP32-mixed run")
+    ap.add_argument("--precision", default=None, choices=["fp64", "fp32-mixed"],
+                    help="profile one precision only (prints no JSON line)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="run a plain device copy of MIB MiB (HBM-counter calibration) and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
+    if args.precision:
+        mk_mesh, mk_flow = gridgen.WORKLOADS[args.workload]
+        mesh, flow = mk_mesh(), mk_flow()
+        el, _, _, bad, kt = run_gpu(mesh, flow, args.precision, args.steps, args.warmup)
+        print(f"bench: {args.precision} {mesh.neles * args.steps / el / 1e6:.1f} Mcell-iters/s "
+              f"kernels(ms) {np.round(kt, 5).tolist()}", flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); multi-GPU "
               "sharding is a design, not an implementation", file=sys.stderr)

@@ -100,6 +100,43 @@ def run_case(name, mesh, flow, nrun, dump, sample_stride=1, crosscheck=True):
         shutil.rmtree(work)
 
 
+def parse_tecplot(path, nods, neles):
+    """Numbers of a tec_plot.dat: NODS x 8 node values, then NELES x 4 node ids
+    (list-directed records may wrap, so the body is read as one token stream)."""
+    lines = open(path).read().splitlines()
+    head = lines[:2]
+    toks = " ".join(lines[2:]).split()
+    nodes = np.array([float(t) for t in toks[:8 * nods]]).reshape(nods, 8)
+    elems = np.array([int(t) for t in toks[8 * nods:8 * nods + 4 * neles]],
+                     dtype=np.int32).reshape(neles, 4)
+    return head, nodes, elems
+
+
+def run_host_case(name, case_dir, niter):
+    """The reference executable's own outputs: stdout, fort.32, restart.in, tec_plot.dat."""
+    work = tempfile.mkdtemp(prefix=f"tdt_golden_{name}_")
+    try:
+        flow, mesh = caseio.load_case(case_dir)
+        flow = dataclasses.replace(flow, niter=niter, istart=0)
+        caseio.write_case(work, flow, mesh)
+        out = subprocess.run([os.path.join(REF, "turbnstwod")], cwd=work, check=True,
+                             capture_output=True, text=True).stdout
+        c, rtred, npseudo, rinit, niterold = caseio.read_restart(os.path.join(work, "restart.in"),
+                                                                 mesh.neles)
+        head, nodes, elems = parse_tecplot(os.path.join(work, "tec_plot.dat"), mesh.nods,
+                                           mesh.neles)
+        data = {f"m_{k}": v for k, v in mesh.arrays().items()}
+        data.update({
+            "flow_json": np.array(json.dumps(dataclasses.asdict(flow))), "nrun": np.array(niter),
+            "stdout": np.array(out), "fort32": np.array(open(os.path.join(work, "fort.32")).read()),
+            "restart_c": c, "restart_tail": np.array([rtred, npseudo, rinit, niterold]),
+            "tec_head": np.array("\n".join(head)), "tec_nodes": nodes, "tec_elems": elems})
+        np.savez_compressed(os.path.join(OUT, name + ".npz"), **data)
+        print(f"{name}: reference executable outputs after {niter} iterations")
+    finally:
+        shutil.rmtree(work)
+
+
 def tiny_duct_radius(x):
     return gridgen.cosine_duct_radius(x, r_in=5.0, r_throat=2.5, x_throat=15.0, r_exit=4.0,
                                       x_exit=40.0)
@@ -136,6 +173,7 @@ def main():
         flow, mesh = caseio.load_case(c1)
         run_case("case1", mesh, flow, nrun=200, dump=False)
         run_case("case1_axi", mesh, dataclasses.replace(flow, iaxis=1), nrun=100, dump=False)
+        run_host_case("case1_host1000", c1, niter=1000)
     # the 385 x 129 axisymmetric duct workload, 40 iterations, sampled final state
     m = gridgen.duct()
     f = gridgen.flow_duct()
