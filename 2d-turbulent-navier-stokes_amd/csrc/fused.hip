@@ -26,6 +26,10 @@
 
 #include <hip/hip_ext.h>
 
+#include <algorithm>
+#include <cstdlib>
+#include <string>
+
 namespace tdt {
 
 struct FDev {
@@ -61,6 +65,8 @@ struct FusedBufs {
     int chunk = 8;
     hipEvent_t ev[12];
     bool have_events = false;
+    bool quad = false;               // face-parallel kernels (4 lanes per cell)
+    int nbq_cells = 0, nbq_nodes = 0;
 };
 
 namespace {
@@ -393,6 +399,14 @@ __global__ void __launch_bounds__(kBlock) k_stage(FDev F, Consts<double> K, Cons
 
 __global__ void __launch_bounds__(kBlock) k_finalize(FDev F, int nblk) { fold_monitor(F, nblk); }
 
+}  // namespace
+}  // namespace tdt
+
+#include "quad_kernels.hpp"
+
+namespace tdt {
+namespace {
+
 __global__ void k_reset_counters(int *ctr, unsigned long long *dtslot) {
     ctr[0] = 0;
     ctr[1] = 0;
@@ -421,6 +435,19 @@ void launch(tdt_ctx *c, FusedBufs &B, int slot, K kern, unsigned grid, Args... a
 template <class R, class SS, class SD>
 hipError_t enqueue_iteration(tdt_ctx *c, FusedBufs &B, const Consts<R> &KR) {
     const FDev &F = B.F;
+    if (B.quad) {
+        const unsigned nq = B.nbq_cells;
+        launch(c, B, 0, kq_pre<SD, R>, nq + B.nbq_nodes, F, c->K, KR, B.nbq_cells, B.nbq_cells);
+        if (c->profiling && B.have_events) {   // no separate dissipation/viscous kernel
+            (void)hipEventRecord(B.ev[2], c->stream);
+            (void)hipEventRecord(B.ev[3], c->stream);
+        }
+        launch(c, B, 2, kq_stage<R, double, SS, SD, 1>, nq, F, c->K, KR, 0.25);
+        launch(c, B, 3, kq_stage<R, SS, SS, SD, 2>, nq, F, c->K, KR, 1.0 / 3.0);
+        launch(c, B, 4, kq_stage<R, SS, SS, SD, 3>, nq, F, c->K, KR, 0.5);
+        launch(c, B, 5, kq_stage<R, SS, double, SD, 4>, nq, F, c->K, KR, 1.0);
+        return hipGetLastError();
+    }
     const unsigned nbc = B.nb_cells;
     launch(c, B, 0, k_pre<SD, R>, nbc + B.nb_nodes, F, c->K, KR, B.nb_cells);
     launch(c, B, 1, k_dv<SD, R>, nbc, F, c->K, KR);
@@ -480,9 +507,13 @@ int fused_setup(tdt_ctx *c) {
     }
     B->nb_cells = (int)nblocks(D.n);
     B->nb_nodes = (int)nblocks(D.m);
+    B->nbq_cells = (D.n + kBlock / 4 - 1) / (kBlock / 4);
+    B->nbq_nodes = (D.m + kBlock / 4 - 1) / (kBlock / 4);
+    const char *kv = std::getenv("TDT_KERNELS");
+    B->quad = kv && std::string(kv) == "quad";
     F.dtslot = dalloc<unsigned long long>(*B, 1);
     F.ctr = dalloc<int>(*B, 4);
-    F.part = dalloc<double>(*B, (size_t)B->nb_cells * kPartStride);
+    F.part = dalloc<double>(*B, (size_t)std::max(B->nb_cells, B->nbq_cells) * kPartStride);
     ok = ok && F.dtslot && F.ctr && F.part;
     if (ok)
         for (int k = 0; k < 12; ++k) ok = ok && hipEventCreate(&B->ev[k]) == hipSuccess;
@@ -576,7 +607,8 @@ int fused_iterate(tdt_ctx *c, int niter, double *l2, double *linf, int *first_no
         }
     }
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, c->stream, F, B.nb_cells);
+        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, c->stream, F,
+                           B.quad ? B.nbq_cells : B.nb_cells);
         hipLaunchKernelGGL(k_cold_from_qn, dim3(nblocks(F.n)), dim3(kBlock), 0, c->stream, F,
                            c->D.cold);
     }

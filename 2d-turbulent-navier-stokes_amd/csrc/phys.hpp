@@ -8,6 +8,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "tmath.hpp"
+
 namespace tdt {
 
 // Model and scheme constants (constants.f90:15-45).
@@ -39,9 +41,11 @@ __device__ __forceinline__ R dmax(R a, R b) { return a > b ? a : b; }
 template <class R>
 __device__ __forceinline__ R dmin(R a, R b) { return a < b ? a : b; }
 
-__device__ __forceinline__ double tpow(double a, double b) { return pow(a, b); }
+// FP64: the branch-free functions of tmath.hpp (a few ulps of glibc); FP32: the
+// device library's
+__device__ __forceinline__ double tpow(double a, double b) { return tm::pow(a, b); }
 __device__ __forceinline__ float tpow(float a, float b) { return powf(a, b); }
-__device__ __forceinline__ double ttanh(double a) { return tanh(a); }
+__device__ __forceinline__ double ttanh(double a) { return tm::tanh(a); }
 __device__ __forceinline__ float ttanh(float a) { return tanhf(a); }
 __device__ __forceinline__ double tsqrt(double a) { return sqrt(a); }
 __device__ __forceinline__ float tsqrt(float a) { return sqrtf(a); }
