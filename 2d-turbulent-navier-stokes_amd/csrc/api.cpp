@@ -446,6 +446,13 @@ int tdt_kernel_times(tdt_ctx *c, double *times_ms) {
     return TDT_OK;
 }
 
+int tdt_kernel_set(tdt_ctx *c, int32_t *set) {
+    CHECK_CTX(c);
+    if (!set) return TDT_ERR_ARG;
+    *set = tdt::fused_variant(c);
+    return TDT_OK;
+}
+
 int tdt_synchronize(tdt_ctx *c) {
     CHECK_CTX(c);
     return sync(c);

@@ -1,0 +1,72 @@
+// Shared definitions of the fused hot path: device view, host-side buffers
+// and the per-precision enqueue functions (fused.hip: FP64, strict IEEE;
+// fused_mixed.hip: FP32-mixed, compiled with fast reciprocal / approximate
+// library functions).
+#pragma once
+
+#include "ctx.hpp"
+#include "reduce.hpp"
+
+#include <hip/hip_ext.h>
+
+#include <algorithm>
+#include <cstdlib>
+#include <string>
+#include <vector>
+
+namespace tdt {
+
+struct FDev {
+    int n, g, m, nt;
+    int remap;                      // XCD-aware block order
+    const int4 *nc4, *nod4;
+    const double *xn, *yn;
+    const int *gparent, *gtype, *gopp;
+    const int *nptr, *nlist;
+    const int4 *novr;
+    const double *xcc, *ycc, *vol, *dl, *sc;
+    double *qn;                     // [5][nt] Q^n (the C array), FP64
+    void *st[2];                    // [5][n] stage states Q(1), Q(2)/Q(3)
+    double *dt;                     // [n]
+    void *anu, *anurt;              // [n]  (storage type SD)
+    void *d2q;                      // [5][nt]
+    void *dd;                       // [5][n] dissipation D
+    void *vis;                      // [4][n]
+    void *node;                     // [5][m] UN VN TN RHON RN
+    unsigned long long *dtslot;     // DTMIN bits for LOCAL = 0
+    double *part;                   // per-block monitor partials
+    double *hist;                   // [niter][10]
+    int *ctr;                       // 0: iteration, 2: first non-finite iteration,
+                                    // 3: stage-4 monitor partials pending
+};
+
+struct FusedBufs {
+    FDev F{};
+    int nb_cells = 0, nb_nodes = 0;
+    size_t hist_cap = 0;
+    std::vector<void *> own;
+    hipGraph_t graph1 = nullptr, graphK = nullptr;
+    hipGraphExec_t exec1 = nullptr, execK = nullptr;
+    int chunk = 8;
+    hipEvent_t ev[12];
+    bool have_events = false;
+    int variant = 0;                 // kVariant*: kernel set of the iteration
+    bool tuned = false;              // variant picked by timing (autotune)
+    double *qbak = nullptr;          // Q^n saved while candidates are timed
+    int nbq_cells = 0, nbq_nodes = 0;
+};
+
+// Kernel sets of one iteration; all give bitwise-identical results.
+constexpr int kVariantMerged = 0;  // k_pre, k_rk x4 (stage 1 forms D and VIS)
+constexpr int kVariantSplit = 1;   // k_pre, k_dv, k_rk x4
+constexpr int kVariantQuad = 2;    // kq_pre, kq_stage x4 (four lanes per cell)
+constexpr int kNumVariants = 3;
+// trial iterations per kernel set when the first tdt_iterate call picks one
+constexpr int kTuneIterations = 4;
+
+
+// One iteration's kernels enqueued on the context's stream (FP64 / FP32-mixed).
+hipError_t enqueue_fp64(tdt_ctx *c, FusedBufs &B);
+hipError_t enqueue_mixed(tdt_ctx *c, FusedBufs &B);
+
+}  // namespace tdt

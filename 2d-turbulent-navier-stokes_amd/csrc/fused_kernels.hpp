@@ -1,0 +1,481 @@
+// Device code of the fused hot path, included by fused.hip and fused_mixed.hip
+// (each instantiates one precision).  See fused.hip for the kernel sequence.
+#pragma once
+
+#include "fused.hpp"
+#include "quad.hpp"
+
+namespace tdt {
+namespace {
+
+// Logical block index so that each XCD (blocks are dispatched round-robin over
+// the 8 XCDs) works on a contiguous range of cells and its L2 sees the halo.
+__device__ __forceinline__ int logical_block(int remap) {
+    int b = blockIdx.x, nb = gridDim.x;
+    if (!remap || nb < 16) return b;
+    int xcd = b & 7, local = b >> 3, per = nb >> 3, rem = nb & 7;
+    return xcd < rem ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
+}
+
+template <class T>
+__device__ __forceinline__ double ld(const void *p, size_t i) {
+    return (double)static_cast<const T *>(p)[i];
+}
+template <class T>
+__device__ __forceinline__ void stv(void *p, size_t i, double v) {
+    static_cast<T *>(p)[i] = (T)v;
+}
+
+// Folds the pending stage-4 partials of the previous iteration into the
+// history (one block; block 0 of k_pre and k_finalize call it).  The kernel
+// boundary orders the partial writes before these reads.
+__device__ __forceinline__ void fold_monitor(const FDev &F, int nblk) {
+    if (F.ctr[3] == 0) return;   // uniform across the block
+    __shared__ int flag;
+    if (threadIdx.x == 0) flag = 0;
+    __syncthreads();
+    int it = F.ctr[0];
+    reduce_partials(F.part, nblk, F.n, F.hist + (size_t)it * 10, &flag);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (flag && F.ctr[2] == 0) F.ctr[2] = it + 1;
+        F.ctr[0] = it + 1;
+        F.ctr[3] = 0;
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// k_pre: cells [0, nb_cells) blocks, nodes after.
+template <class SD, class R>
+__global__ void __launch_bounds__(kBlock) k_pre(FDev F, Consts<double> K, Consts<R> KR,
+                                                int nb_cells) {
+    int b = blockIdx.x;
+    if (b == 0) fold_monitor(F, nb_cells);
+    if (b < nb_cells) {
+        int i = b * kBlock + threadIdx.x;
+        double dtl = 100000.0;
+        if (i < F.n) {
+            const int nt = F.nt;
+            const double *Q = F.qn;
+            double c0 = Q[i], c1 = Q[nt + i], c2 = Q[2 * nt + i], c3 = Q[3 * nt + i],
+                   c4 = Q[4 * nt + i];
+            double u, v, t, p;
+            uvtp_from_q(c0, c1, c2, c3, u, v, t, p, K);
+            double vel = sqrt(u * u + v * v);
+            double dt = K.cfl * F.dl[i] / (fabs(vel) + sqrt(K.gamma_rr * t));
+            if (K.local) F.dt[i] = dt;
+            dtl = dt;
+            int4 nb4 = F.nc4[i];
+            int nb[4] = {nb4.x, nb4.y, nb4.z, nb4.w};
+            // ghosts hold parent copies in DISSIPATION (solver_state.f90:77-88)
+            double pn[4], cn[5][4];
+            for (int k = 0; k < 4; ++k) {
+                int j = nb[k] < F.n ? nb[k] : i;
+                double q0 = Q[j], q1 = Q[nt + j], q2 = Q[2 * nt + j], q3 = Q[3 * nt + j];
+                double uj, vj, tj, pj;
+                uvtp_from_q(q0, q1, q2, q3, uj, vj, tj, pj, K);
+                pn[k] = pj;
+                cn[0][k] = q0; cn[1][k] = q1; cn[2][k] = q2; cn[3][k] = q3;
+                cn[4][k] = Q[4 * nt + j];
+            }
+            double anun = fabs(pn[0] - p) + fabs(pn[1] - p) + fabs(pn[2] - p) + fabs(pn[3] - p);
+            double anud = fabs(pn[0] + p) + fabs(pn[1] + p) + fabs(pn[2] + p) + fabs(pn[3] + p);
+            stv<SD>(F.anu, i, anun / anud);
+            anun = fabs(cn[4][0] - c4) + fabs(cn[4][1] - c4) + fabs(cn[4][2] - c4) +
+                   fabs(cn[4][3] - c4);
+            anud = fabs(cn[4][0] + c4) + fabs(cn[4][1] + c4) + fabs(cn[4][2] + c4) +
+                   fabs(cn[4][3] + c4);
+            stv<SD>(F.anurt, i, anun / dmax(anud, 8.0 * c0 * K.rtol));
+            const double ci[5] = {c0, c1, c2, c3, c4};
+            for (int m = 0; m < 5; ++m)
+                stv<SD>(F.d2q, (size_t)m * nt + i,
+                        cn[m][0] + cn[m][1] + cn[m][2] + cn[m][3] - 4.0 * ci[m]);
+        }
+        if (!K.local) block_atomic_min_pos(dtl, reinterpret_cast<double *>(F.dtslot));
+        return;
+    }
+    int v = (b - nb_cells) * kBlock + threadIdx.x;
+    if (v >= F.m) return;
+    const int nt = F.nt;
+    const double *Q = F.qn;
+    int4 ov = F.novr[v];
+    int j0 = F.nptr[v], j1 = F.nptr[v + 1];
+    R au = R(0), av = R(0), at = R(0), arho = R(0), ar = ov.w ? KR.rtol_rtred : R(0);
+    for (int j = j0; j < j1; ++j) {
+        int id = F.nlist[j];
+        R u, vv, t, rho, c5;
+        if (id < F.n) {
+            R p;
+            rho = (R)Q[id];
+            uvtp_from_q<R>(rho, (R)Q[nt + id], (R)Q[2 * nt + id], (R)Q[3 * nt + id], u, vv, t, p, KR);
+            c5 = (R)Q[4 * nt + id];
+        } else {
+            int k = id - F.n, par = F.gparent[k], opp = F.gopp[k];
+            Prim<R> pp, s;
+            prim_from_q<R>(pp, (R)Q[par], (R)Q[nt + par], (R)Q[2 * nt + par], (R)Q[3 * nt + par],
+                           (R)Q[4 * nt + par], KR);
+            R uo = R(0), vo = R(0), to = R(0), po = R(0);
+            int ty = F.gtype[k];
+            if (ty == 3)
+                uvtp_from_q<R>((R)Q[opp], (R)Q[nt + opp], (R)Q[2 * nt + opp], (R)Q[3 * nt + opp],
+                               uo, vo, to, po, KR);
+            ghost_state<R>(s, ty, pp, uo, vo, to, po, (R)F.ycc[id], KR);
+            u = s.u; vv = s.v; t = s.t; rho = s.c[0]; c5 = s.c[4];
+        }
+        R rr = c5 / rho;
+        if (j >= ov.x) au += u;
+        if (j >= ov.y) av += vv;
+        at += t;
+        arho += rho;
+        if (j >= ov.z) ar += rr;
+    }
+    R cnt = (R)(j1 - j0);
+    const int m = F.m;
+    stv<SD>(F.node, v, au / cnt);
+    stv<SD>(F.node, (size_t)m + v, av / cnt);
+    stv<SD>(F.node, (size_t)2 * m + v, at / cnt);
+    stv<SD>(F.node, (size_t)3 * m + v, arho / cnt);
+    stv<SD>(F.node, (size_t)4 * m + v, dmax(KR.rtol, ar / cnt));
+}
+
+// ---------------------------------------------------------------------------------
+// k_dv: dissipation D and viscous fluxes VIS from Q^n in a kernel of their own
+// (the split variant; the merged variant forms them inside stage 1).
+template <class SD, class R>
+__global__ void __launch_bounds__(kBlock) k_dv(FDev F, Consts<double> K, Consts<R> KR) {
+    int i = logical_block(F.remap) * kBlock + threadIdx.x;
+    if (i >= F.n) return;
+    const int nt = F.nt, n = F.n, m = F.m;
+    const double *Q = F.qn;
+    double dti = K.local ? F.dt[i] : __longlong_as_double((long long)*F.dtslot);
+    if (!K.local) F.dt[i] = dti;
+    int4 nb4 = F.nc4[i], nd4 = F.nod4[i];
+    const int nb[4] = {nb4.x, nb4.y, nb4.z, nb4.w};
+    const int nd[4] = {nd4.x, nd4.y, nd4.z, nd4.w};
+    // ---- dissipation (solver_state.f90:183-223); ghost neighbour = parent copy
+    double ci[5];
+    for (int q = 0; q < 5; ++q) ci[q] = Q[q * nt + i];
+    R voli = (R)F.vol[i];
+    R volbydt = voli / (R)dti;
+    R anui = (R)ld<SD>(F.anu, i), anurti = (R)ld<SD>(F.anurt, i);
+    R d2qi[5];
+    for (int q = 0; q < 5; ++q) d2qi[q] = (R)ld<SD>(F.d2q, (size_t)q * nt + i);
+    R d2[5], d4[5];
+    for (int k = 0; k < 4; ++k) {
+        int j = nb[k] < n ? nb[k] : i;
+        R dtj = K.local ? (R)F.dt[j] : (R)dti;
+        R volj = (R)F.vol[j];
+        R a2, a4, b2, b4;
+        jst_coeffs<R>(anui, (R)ld<SD>(F.anu, j), volbydt, volj, dtj, a2, a4);
+        jst_coeffs<R>(anurti, (R)ld<SD>(F.anurt, j), volbydt, volj, dtj, b2, b4);
+        for (int q = 0; q < 5; ++q) {
+            R c2 = q < 4 ? a2 : b2, c4 = q < 4 ? a4 : b4;
+            R dc = (R)(Q[q * nt + j] - ci[q]);
+            R dq = (R)ld<SD>(F.d2q, (size_t)q * nt + j) - d2qi[q];
+            if (k == 0) {
+                d2[q] = c2 * dc;
+                d4[q] = c4 * dq;
+            } else {
+                d2[q] = d2[q] + c2 * dc;
+                d4[q] = d4[q] + c4 * dq;
+            }
+        }
+    }
+    for (int q = 0; q < 5; ++q) stv<SD>(F.dd, (size_t)q * n + i, d2[q] - d4[q]);
+    // ---- viscous fluxes (viscous.f90:62-70, 130-217)
+    R ua, va, ta, pa;
+    R rhoa = (R)ci[0];
+    uvtp_from_q<R>(rhoa, (R)ci[1], (R)ci[2], (R)ci[3], ua, va, ta, pa, KR);
+    R ra = (R)ci[4] / rhoa;
+    R xa = (R)F.xcc[i], ya = (R)F.ycc[i];
+    Prim<R> pi;
+    bool have_pi = false;
+    R vu = R(0), vv = R(0), vt = R(0), vr = R(0);
+    for (int f = 0; f < 4; ++f) {
+        int bnode = nd[f], dnode = nd[(f + 1) & 3], c = nb[f];
+        R uc, vc, tc, rhoc, c5c;
+        if (c < n) {
+            R pc;
+            rhoc = (R)Q[c];
+            uvtp_from_q<R>(rhoc, (R)Q[nt + c], (R)Q[2 * nt + c], (R)Q[3 * nt + c], uc, vc, tc, pc, KR);
+            c5c = (R)Q[4 * nt + c];
+        } else {
+            if (!have_pi) {
+                prim_from_q<R>(pi, rhoa, (R)ci[1], (R)ci[2], (R)ci[3], (R)ci[4], KR);
+                have_pi = true;
+            }
+            int k = c - n, ty = F.gtype[k];
+            R uo = R(0), vo = R(0), to = R(0), po = R(0);
+            if (ty == 3) {
+                int o = F.gopp[k];
+                uvtp_from_q<R>((R)Q[o], (R)Q[nt + o], (R)Q[2 * nt + o], (R)Q[3 * nt + o], uo, vo,
+                               to, po, KR);
+            }
+            Prim<R> s;
+            ghost_state<R>(s, ty, pi, uo, vo, to, po, (R)F.ycc[c], KR);
+            uc = s.u; vc = s.v; tc = s.t; rhoc = s.c[0]; c5c = s.c[4];
+        }
+        R rc = c5c / rhoc;
+        ViscFace<R> o = viscous_face<R>(
+            (R)F.sc[(2 * f) * n + i], (R)F.sc[(2 * f + 1) * n + i], xa, ya, (R)F.xn[bnode],
+            (R)F.yn[bnode], (R)F.xcc[c], (R)F.ycc[c], (R)F.xn[dnode], (R)F.yn[dnode], ua, va, ta,
+            rhoa, ra, (R)ld<SD>(F.node, bnode), (R)ld<SD>(F.node, (size_t)m + bnode),
+            (R)ld<SD>(F.node, (size_t)2 * m + bnode), (R)ld<SD>(F.node, (size_t)3 * m + bnode),
+            (R)ld<SD>(F.node, (size_t)4 * m + bnode), uc, vc, tc, rhoc, rc,
+            (R)ld<SD>(F.node, dnode), (R)ld<SD>(F.node, (size_t)m + dnode),
+            (R)ld<SD>(F.node, (size_t)2 * m + dnode), (R)ld<SD>(F.node, (size_t)3 * m + dnode),
+            (R)ld<SD>(F.node, (size_t)4 * m + dnode), KR);
+        vu = vu + o.u0 + o.u1;
+        vv = vv + o.v0 + o.v1;
+        vt = vt + o.t0 + o.t1;
+        vr = vr + o.r0 + o.r1;
+    }
+    stv<SD>(F.vis, i, vu);
+    stv<SD>(F.vis, (size_t)n + i, vv);
+    stv<SD>(F.vis, (size_t)2 * n + i, vt);
+    stv<SD>(F.vis, (size_t)3 * n + i, vr);
+}
+
+// ---------------------------------------------------------------------------------
+// k_rk: one RK stage for one cell per lane.  Stage 1 also forms the dissipation D
+// (solver_state.f90:120-223) and viscous fluxes VIS (viscous.f90:62-217) from
+// Q^n and stores them for stages 2-4 (they stay frozen, as in the reference).
+// The faces are visited once, in order: the neighbour's primitives (or its ghost,
+// built from this cell), the face's flux terms, its viscous terms (stage 1) and
+// its share of the source-term gradient differences.  SI / SO: stage input /
+// output storage types; SD: storage of D, VIS, sensors and nodal averages.
+// DV = false: stage 1 reads D and VIS written by k_dv (the split variant).
+template <class R, class SI, class SO, class SD, int STAGE, bool DV = true>
+__global__ void __launch_bounds__(kBlock) k_rk(FDev F, Consts<double> K, Consts<R> KR,
+                                               double alpha) {
+    constexpr bool kInlineDV = STAGE == 1 && DV;
+    const int i = logical_block(F.remap) * kBlock + threadIdx.x;
+    const int nt = F.nt, n = F.n;
+    const void *qin = STAGE == 1 ? (const void *)F.qn : (const void *)F.st[STAGE & 1];
+    const size_t sin = STAGE == 1 ? (size_t)nt : (size_t)n;
+    double cnew[5] = {0, 0, 0, 0, 0}, cold[5] = {0, 0, 0, 0, 0};
+    const bool active = i < n;
+    if (active) {
+        const int4 nb4 = F.nc4[i];
+        const double *Q = F.qn;
+        double dti;
+        if (STAGE == 1) {
+            dti = K.local ? F.dt[i] : __longlong_as_double((long long)*F.dtslot);
+            if (!K.local) F.dt[i] = dti;
+        } else {
+            dti = F.dt[i];
+        }
+        double dd[5], vs[4];
+        if (kInlineDV) {
+            // dissipation faces; a ghost neighbour is a copy of this cell
+            double ci[5];
+            R d2qi[5];
+#pragma unroll
+            for (int q = 0; q < 5; ++q) {
+                ci[q] = Q[q * nt + i];
+                d2qi[q] = (R)ld<SD>(F.d2q, (size_t)q * nt + i);
+            }
+            const R anui = (R)ld<SD>(F.anu, i), anurti = (R)ld<SD>(F.anurt, i);
+            const R volbydt = (R)F.vol[i] / (R)dti;
+            R d2[5], d4[5];
+#pragma unroll
+            for (int f = 0; f < 4; ++f) {
+                const int jn = sel4(f, nb4.x, nb4.y, nb4.z, nb4.w);
+                const int j = jn < n ? jn : i;
+                const R dtj = K.local ? (R)F.dt[j] : (R)dti;
+                const R volj = (R)F.vol[j];
+                R a2, a4, b2, b4;
+                jst_coeffs<R>(anui, (R)ld<SD>(F.anu, j), volbydt, volj, dtj, a2, a4);
+                jst_coeffs<R>(anurti, (R)ld<SD>(F.anurt, j), volbydt, volj, dtj, b2, b4);
+#pragma unroll
+                for (int q = 0; q < 5; ++q) {
+                    const R t2 = (q < 4 ? a2 : b2) * (R)(Q[q * nt + j] - ci[q]);
+                    const R t4 = (q < 4 ? a4 : b4) * ((R)ld<SD>(F.d2q, (size_t)q * nt + j) - d2qi[q]);
+                    d2[q] = f == 0 ? t2 : d2[q] + t2;
+                    d4[q] = f == 0 ? t4 : d4[q] + t4;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 5; ++q) {
+                dd[q] = (double)(SD)(d2[q] - d4[q]);
+                stv<SD>(F.dd, (size_t)q * n + i, dd[q]);
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < 5; ++q) dd[q] = ld<SD>(F.dd, (size_t)q * n + i);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) vs[q] = ld<SD>(F.vis, (size_t)q * n + i);
+        }
+
+        Prim<R> pi;
+        prim_from_q<R>(pi, (R)ld<SI>(qin, i), (R)ld<SI>(qin, sin + i), (R)ld<SI>(qin, 2 * sin + i),
+                       (R)ld<SI>(qin, 3 * sin + i), (R)ld<SI>(qin, 4 * sin + i), KR);
+        R acc[5];
+        R vu = R(0), vv = R(0), vt = R(0), vr = R(0);
+        SrcNbr<R> sa{}, sb{};
+        SrcDiff<R> sd{};
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+            const int jn = sel4(f, nb4.x, nb4.y, nb4.z, nb4.w);
+            Prim<R> pj;
+            if (jn < n) {
+                prim_from_q<R>(pj, (R)ld<SI>(qin, jn), (R)ld<SI>(qin, sin + jn),
+                               (R)ld<SI>(qin, 2 * sin + jn), (R)ld<SI>(qin, 3 * sin + jn),
+                               (R)ld<SI>(qin, 4 * sin + jn), KR);
+            } else {
+                // the ghost across this face: only this cell reads it (boundary.f90)
+                const int k = jn - n, ty = F.gtype[k];
+                R uo = R(0), vo = R(0), to = R(0), po = R(0);
+                if (ty == 3) {
+                    const int o = sel4((f + 2) & 3, nb4.x, nb4.y, nb4.z, nb4.w);
+                    uvtp_from_q<R>((R)ld<SI>(qin, o), (R)ld<SI>(qin, sin + o),
+                                   (R)ld<SI>(qin, 2 * sin + o), (R)ld<SI>(qin, 3 * sin + o), uo,
+                                   vo, to, po, KR);
+                }
+                ghost_state<R>(pj, ty, pi, uo, vo, to, po, (R)F.ycc[jn], KR);
+            }
+            // central flux terms of this face (residual.f90:34-54)
+            const R sx = (R)F.sc[(2 * f) * n + i], sy = (R)F.sc[(2 * f + 1) * n + i];
+#pragma unroll
+            for (int q = 0; q < 5; ++q) {
+                const R tf = (pi.f[q] + pj.f[q]) * sx;
+                const R tg = (pi.g[q] + pj.g[q]) * sy;
+                acc[q] = f == 0 ? tf + tg : acc[q] + tf + tg;
+            }
+            const R xj = (R)F.xcc[jn], yj = (R)F.ycc[jn];
+            const SrcNbr<R> nbv = src_nbr<R>(xj, yj, pj.u, pj.v, pj.c[0], pj.c[4]);
+            if (f == 0) sa = nbv;
+            if (f == 1) sb = nbv;
+            if (f == 2) src_pair(sa, nbv, sd.xac, sd.yac, sd.qac, sd.rac, sd.uac, sd.vac);
+            if (f == 3) src_pair(sb, nbv, sd.xbd, sd.ybd, sd.qbd, sd.rbd, sd.ubd, sd.vbd);
+            if (kInlineDV) {
+                const int4 nd4 = F.nod4[i];
+                const int bn = sel4(f, nd4.x, nd4.y, nd4.z, nd4.w);
+                const int dn = sel4(f, nd4.y, nd4.z, nd4.w, nd4.x);
+                const int m = F.m;
+                const ViscFace<R> o = viscous_face<R>(
+                    sx, sy, (R)F.xcc[i], (R)F.ycc[i], (R)F.xn[bn], (R)F.yn[bn], xj, yj,
+                    (R)F.xn[dn], (R)F.yn[dn], pi.u, pi.v, pi.t, pi.c[0], pi.c[4] / pi.c[0],
+                    (R)ld<SD>(F.node, bn), (R)ld<SD>(F.node, (size_t)m + bn),
+                    (R)ld<SD>(F.node, (size_t)2 * m + bn), (R)ld<SD>(F.node, (size_t)3 * m + bn),
+                    (R)ld<SD>(F.node, (size_t)4 * m + bn), pj.u, pj.v, pj.t, pj.c[0],
+                    pj.c[4] / pj.c[0], (R)ld<SD>(F.node, dn), (R)ld<SD>(F.node, (size_t)m + dn),
+                    (R)ld<SD>(F.node, (size_t)2 * m + dn), (R)ld<SD>(F.node, (size_t)3 * m + dn),
+                    (R)ld<SD>(F.node, (size_t)4 * m + dn), KR);
+                vu = vu + o.u0 + o.u1;
+                vv = vv + o.v0 + o.v1;
+                vt = vt + o.t0 + o.t1;
+                vr = vr + o.r0 + o.r1;
+            }
+        }
+        if (kInlineDV) {
+            vs[0] = (double)(SD)vu; vs[1] = (double)(SD)vv;
+            vs[2] = (double)(SD)vt; vs[3] = (double)(SD)vr;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) stv<SD>(F.vis, (size_t)q * n + i, vs[q]);
+        }
+        R s3;
+        const R s5 = source_terms_d<R>(sd, pi.c[0], pi.c[4], pi.t, pi.p, pi.v, (R)F.ycc[i], s3, KR);
+        // residual.f90:34-60 and the update residual.f90:76-81 (FP64)
+        const double vol = F.vol[i];
+        double r[5];
+#pragma unroll
+        for (int q = 0; q < 5; ++q) r[q] = (double)(R(0.5) * acc[q]);
+        r[4] = r[4] - vol * (double)s5;
+        if (K.iaxis == 1) r[2] = r[2] - vol * (double)s3;
+        const double a = alpha * (dti / vol);
+#pragma unroll
+        for (int q = 0; q < 5; ++q) cold[q] = Q[q * nt + i];
+        cnew[0] = cold[0] - a * (r[0] - dd[0]);
+        cnew[1] = cold[1] - a * (r[1] - vs[0] - dd[1]);
+        cnew[2] = cold[2] - a * (r[2] - vs[1] - dd[2]);
+        cnew[3] = cold[3] - a * (r[3] - vs[2] - dd[3]);
+        cnew[4] = cold[4] - a * kCflTurb * (r[4] - vs[3] - dd[4]);
+        cnew[4] = cnew[0] * dmax(K.rtol, cnew[4] / cnew[0]);
+        if (STAGE < 4) {
+            void *qo = F.st[(STAGE - 1) & 1];
+#pragma unroll
+            for (int q = 0; q < 5; ++q) stv<SO>(qo, (size_t)q * n + i, cnew[q]);
+        } else {
+            double *Qw = F.qn;
+#pragma unroll
+            for (int q = 0; q < 5; ++q) Qw[q * nt + i] = cnew[q];
+        }
+    }
+    if (STAGE == 2 && !K.local && blockIdx.x == 0 && threadIdx.x == 0)
+        *F.dtslot = (unsigned long long)__double_as_longlong(100000.0);
+    if (STAGE == 4) {
+        // per-iteration L2 / Linf of C - COLD and the non-finite flag: block
+        // partials only; the next pre-pass (or k_finalize) folds them, so no
+        // grid-wide fence sits in this kernel
+        double sq[5], mx[5];
+        int bad = 0;
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+            const double d = cnew[q] - cold[q];
+            sq[q] = active ? d * d : 0.0;
+            mx[q] = active ? fabs(d) : 0.0;
+            bad |= active && !isfinite(cnew[q]);
+        }
+        block_partials_write(sq, mx, bad, F.part + (size_t)blockIdx.x * kPartStride);
+        if (blockIdx.x == 0 && threadIdx.x == 0) F.ctr[3] = 1;   // partials pending
+    }
+}
+
+}  // namespace
+}  // namespace tdt
+
+#include "quad_kernels.hpp"
+
+namespace tdt {
+namespace {
+
+
+
+// Launches one kernel; when profiling, with start/stop events that the runtime
+// stamps at the kernel's own begin and end (same durations a kernel trace shows).
+template <class K, class... Args>
+void launch(tdt_ctx *c, FusedBufs &B, int slot, K kern, unsigned grid, Args... args) {
+    if (c->profiling && B.have_events)
+        hipExtLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, c->stream, B.ev[2 * slot],
+                              B.ev[2 * slot + 1], 0, args...);
+    else
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, c->stream, args...);
+}
+
+template <class R, class SS, class SD>
+hipError_t enqueue_iteration(tdt_ctx *c, FusedBufs &B, const Consts<R> &KR) {
+    const FDev &F = B.F;
+    if (B.variant == kVariantQuad) {
+        const unsigned nq = B.nbq_cells;
+        launch(c, B, 0, kq_pre<SD, R>, nq + B.nbq_nodes, F, c->K, KR, B.nbq_cells, B.nbq_cells);
+        if (c->profiling && B.have_events) {
+            (void)hipEventRecord(B.ev[2], c->stream);
+            (void)hipEventRecord(B.ev[3], c->stream);
+        }
+        launch(c, B, 2, kq_stage<R, double, SS, SD, 1>, nq, F, c->K, KR, 0.25);
+        launch(c, B, 3, kq_stage<R, SS, SS, SD, 2>, nq, F, c->K, KR, 1.0 / 3.0);
+        launch(c, B, 4, kq_stage<R, SS, SS, SD, 3>, nq, F, c->K, KR, 0.5);
+        launch(c, B, 5, kq_stage<R, SS, double, SD, 4>, nq, F, c->K, KR, 1.0);
+        return hipGetLastError();
+    }
+    const unsigned nbc = B.nb_cells;
+    launch(c, B, 0, k_pre<SD, R>, nbc + B.nb_nodes, F, c->K, KR, B.nb_cells);
+    if (B.variant == kVariantSplit) {
+        launch(c, B, 1, k_dv<SD, R>, nbc, F, c->K, KR);
+        launch(c, B, 2, k_rk<R, double, SS, SD, 1, false>, nbc, F, c->K, KR, 0.25);
+    } else {
+        if (c->profiling && B.have_events) {   // dissipation and viscous live in stage 1
+            (void)hipEventRecord(B.ev[2], c->stream);
+            (void)hipEventRecord(B.ev[3], c->stream);
+        }
+        launch(c, B, 2, k_rk<R, double, SS, SD, 1>, nbc, F, c->K, KR, 0.25);
+    }
+    launch(c, B, 3, k_rk<R, SS, SS, SD, 2>, nbc, F, c->K, KR, 1.0 / 3.0);
+    launch(c, B, 4, k_rk<R, SS, SS, SD, 3>, nbc, F, c->K, KR, 0.5);
+    launch(c, B, 5, k_rk<R, SS, double, SD, 4>, nbc, F, c->K, KR, 1.0);
+    return hipGetLastError();
+}
+
+}  // namespace
+}  // namespace tdt

@@ -1,0 +1,14 @@
+// FP32-mixed instantiation of the fused kernels (fused_kernels.hpp).  This
+// translation unit is compiled with -freciprocal-math -fapprox-func
+// -ffp-contract=fast: FP32 division and square root become the hardware
+// reciprocal / square-root instructions and multiply-adds fuse.  NaN and Inf
+// semantics are kept (the non-finite monitor relies on them).
+#include "fused_kernels.hpp"
+
+namespace tdt {
+
+hipError_t enqueue_mixed(tdt_ctx *c, FusedBufs &B) {
+    return enqueue_iteration<float, float, float>(c, B, c->Kf);
+}
+
+}  // namespace tdt

@@ -41,10 +41,13 @@ __device__ __forceinline__ R dmax(R a, R b) { return a > b ? a : b; }
 template <class R>
 __device__ __forceinline__ R dmin(R a, R b) { return a < b ? a : b; }
 
-// FP64: the branch-free functions of tmath.hpp (a few ulps of glibc); FP32: the
-// device library's
+// FP64: the branch-free functions of tmath.hpp (a few ulps of glibc).  FP32
+// (mixed path): pow as 2^(b log2 a) with the hardware log2 / exp2 (a few ulps),
+// tanh from the device library.
 __device__ __forceinline__ double tpow(double a, double b) { return tm::pow(a, b); }
-__device__ __forceinline__ float tpow(float a, float b) { return powf(a, b); }
+__device__ __forceinline__ float tpow(float a, float b) {
+    return __builtin_amdgcn_exp2f(b * __builtin_amdgcn_logf(a));
+}
 __device__ __forceinline__ double ttanh(double a) { return tm::tanh(a); }
 __device__ __forceinline__ float ttanh(float a) { return tanhf(a); }
 __device__ __forceinline__ double tsqrt(double a) { return sqrt(a); }
@@ -261,24 +264,47 @@ __device__ __forceinline__ ViscFace<R> viscous_face(R sx, R sy, R xa, R ya, R xb
     return o;
 }
 
-// COMPUTE_SOURCE_TERMS for one cell (source_terms.f90:40-101).  Neighbour
-// values a..d = across faces 1..4; r = C5/C1.  Returns SOURCE5, sets src3.
+// Source-term inputs of one cell: the differences between the neighbours
+// across faces 1 and 3 (a - c) and faces 2 and 4 (b - d) of x, y, |V|, nu~, u,
+// v.  The gradients of source_terms.f90:46-60 use only these differences, so a
+// cell can fold its faces in as it visits them.
 template <class R>
-__device__ __forceinline__ R source_terms(R xa, R xb, R xc, R xd, R ya, R yb, R yc, R yd, R ua,
-                                          R ub, R uc, R ud, R va, R vb, R vc, R vd, R ra, R rb,
-                                          R rc, R rd, R rho, R c5, R t, R p, R v, R ycc,
-                                          R &src3, const Consts<R> &K) {
-    R qa = tsqrt(ua * ua + va * va), qb = tsqrt(ub * ub + vb * vb);
-    R qc = tsqrt(uc * uc + vc * vc), qd = tsqrt(ud * ud + vd * vd);
-    R dnr = (xa - xc) * (yb - yd) - (xb - xd) * (ya - yc);
-    R dqx = ((qa - qc) * (yb - yd) - (qb - qd) * (ya - yc)) / dnr;
-    R drx = ((ra - rc) * (yb - yd) - (rb - rd) * (ya - yc)) / dnr;
-    R dux = ((ua - uc) * (yb - yd) - (ub - ud) * (ya - yc)) / dnr;
-    R dvx = ((va - vc) * (yb - yd) - (vb - vd) * (ya - yc)) / dnr;
-    R dqy = (-((qa - qc) * (xb - xd)) + (qb - qd) * (xa - xc)) / dnr;
-    R dry = (-((ra - rc) * (xb - xd)) + (rb - rd) * (xa - xc)) / dnr;
-    R duy = (-((ua - uc) * (xb - xd)) + (ub - ud) * (xa - xc)) / dnr;
-    R dvy = (-((va - vc) * (xb - xd)) + (vb - vd) * (xa - xc)) / dnr;
+struct SrcDiff {
+    R xac, yac, qac, rac, uac, vac;
+    R xbd, ybd, qbd, rbd, ubd, vbd;
+};
+
+// neighbour values across one face: x, y, |V| = sqrt(u^2 + v^2), nu~ = C5/C1, u, v
+template <class R>
+struct SrcNbr {
+    R x, y, q, r, u, v;
+};
+
+template <class R>
+__device__ __forceinline__ SrcNbr<R> src_nbr(R x, R y, R u, R v, R c1, R c5) {
+    return SrcNbr<R>{x, y, tsqrt(u * u + v * v), c5 / c1, u, v};
+}
+
+template <class R>
+__device__ __forceinline__ void src_pair(SrcNbr<R> a, SrcNbr<R> c, R &dx, R &dy, R &dq, R &dr,
+                                         R &du, R &dv) {
+    dx = a.x - c.x; dy = a.y - c.y; dq = a.q - c.q; dr = a.r - c.r; du = a.u - c.u; dv = a.v - c.v;
+}
+
+// COMPUTE_SOURCE_TERMS for one cell (source_terms.f90:40-101).  Returns
+// SOURCE5 and sets src3 (0 unless IAXIS = 1).
+template <class R>
+__device__ __forceinline__ R source_terms_d(const SrcDiff<R> &d, R rho, R c5, R t, R p, R v,
+                                            R ycc, R &src3, const Consts<R> &K) {
+    R dnr = d.xac * d.ybd - d.xbd * d.yac;
+    R dqx = (d.qac * d.ybd - d.qbd * d.yac) / dnr;
+    R drx = (d.rac * d.ybd - d.rbd * d.yac) / dnr;
+    R dux = (d.uac * d.ybd - d.ubd * d.yac) / dnr;
+    R dvx = (d.vac * d.ybd - d.vbd * d.yac) / dnr;
+    R dqy = (-(d.qac * d.xbd) + d.qbd * d.xac) / dnr;
+    R dry = (-(d.rac * d.xbd) + d.rbd * d.xac) / dnr;
+    R duy = (-(d.uac * d.xbd) + d.ubd * d.xac) / dnr;
+    R dvy = (-(d.vac * d.xbd) + d.vbd * d.xac) / dnr;
     R ri = c5 / rho;
     R amu = K.suth_sqrt_amol * tpow(t, R(0.6));
     R chi = rho * ri / (R(kCmu) * amu);
@@ -312,6 +338,17 @@ __device__ __forceinline__ R source_terms(R xa, R xb, R xc, R xd, R ya, R yb, R 
         src3 = R(0);
     }
     return s5;
+}
+
+// The same from the four neighbours' values (a..d = across faces 1..4).
+template <class R>
+__device__ __forceinline__ R source_terms(const SrcNbr<R> &a, const SrcNbr<R> &b,
+                                          const SrcNbr<R> &c, const SrcNbr<R> &dd, R rho, R c5,
+                                          R t, R p, R v, R ycc, R &src3, const Consts<R> &K) {
+    SrcDiff<R> d;
+    src_pair(a, c, d.xac, d.yac, d.qac, d.rac, d.uac, d.vac);
+    src_pair(b, dd, d.xbd, d.ybd, d.qbd, d.rbd, d.ubd, d.vbd);
+    return source_terms_d(d, rho, c5, t, p, v, ycc, src3, K);
 }
 
 // Dissipation face coefficients (solver_state.f90:192-194): returns D2N, D4N.

@@ -301,12 +301,11 @@ __global__ void __launch_bounds__(kBlock) k_source(Dev D, Consts<double> K) {
         int4 nb = D.nc4[i];
         const double *C1 = D.c, *C5 = D.c + 4 * nt;
         double s3;
-        double s5 = source_terms<double>(
-            D.xcc[nb.x], D.xcc[nb.y], D.xcc[nb.z], D.xcc[nb.w], D.ycc[nb.x], D.ycc[nb.y],
-            D.ycc[nb.z], D.ycc[nb.w], D.u[nb.x], D.u[nb.y], D.u[nb.z], D.u[nb.w], D.v[nb.x],
-            D.v[nb.y], D.v[nb.z], D.v[nb.w], C5[nb.x] / C1[nb.x], C5[nb.y] / C1[nb.y],
-            C5[nb.z] / C1[nb.z], C5[nb.w] / C1[nb.w], C1[i], C5[i], D.t[i], D.p[i], D.v[i],
-            D.ycc[i], s3, K);
+        auto nbr = [&](int j) {
+            return src_nbr<double>(D.xcc[j], D.ycc[j], D.u[j], D.v[j], C1[j], C5[j]);
+        };
+        double s5 = source_terms<double>(nbr(nb.x), nbr(nb.y), nbr(nb.z), nbr(nb.w), C1[i], C5[i],
+                                         D.t[i], D.p[i], D.v[i], D.ycc[i], s3, K);
         D.src5[i] = s5;
         D.src3[i] = s3;
     }

@@ -86,6 +86,7 @@ def load(path: str = LIB_PATH):
     L.tdt_set_array.argtypes = [vp, ctypes.c_char_p, _dp, ctypes.c_int32]
     L.tdt_set_profiling.argtypes = [vp, ctypes.c_int32]
     L.tdt_kernel_times.argtypes = [vp, _dp]
+    L.tdt_kernel_set.argtypes = [vp, _ip]
     _lib = L
     return L
 
@@ -234,6 +235,14 @@ class Solver:
         out = np.zeros(6)
         self._check(self.L.tdt_kernel_times(self.h, _d(out)))
         return out
+
+    KERNEL_SETS = ("merged", "split", "quad")
+
+    def kernel_set(self) -> str:
+        """Name of the fused kernel set in use ("" before the first tdt_iterate)."""
+        out = np.zeros(1, dtype=np.int32)
+        self._check(self.L.tdt_kernel_set(self.h, _i(out)))
+        return self.KERNEL_SETS[out[0]] if out[0] >= 0 else ""
 
     def synchronize(self):
         self._check(self.L.tdt_synchronize(self.h))

@@ -31,15 +31,21 @@ import tdt  # noqa: E402
 from tdt import caseio, gridgen  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+DESCRIBE = {"channel_97x49": "2D turbulent channel / flat plate",
+            "duct_385x129": "axisymmetric converging-diverging duct",
+            "ramp_2049x513": "2D supersonic 15-degree compression ramp"}
 KERNELS = ("k_pre", "k_dv", "k_stage1", "k_stage2", "k_stage3", "k_stage4")
 
 
-def alg_bytes_per_cell(precision: str, m_over_n: float, local: int) -> dict:
+def alg_bytes_per_cell(precision: str, m_over_n: float, local: int, kset: str = "split") -> dict:
     """Algorithmic bytes per interior cell for one launch of each fused kernel.
 
     Every array a kernel reads or writes is counted once at its storage width;
     neighbour and node re-reads are cache hits; ghost work (O(perimeter)) is
     neglected.  s = width of stage / dissipation / viscous / node storage.
+    The merged and quad kernel sets have no k_dv launch: stage 1 computes the
+    dissipation and viscous fluxes itself, so it moves k_dv's bytes plus its
+    stage output.
     """
     s = 8 if precision == "fp64" else 4
     q, nc, nod, sc, vol, xy, dt, dl = 40, 16, 16, 64, 8, 16, 8, 8
@@ -51,18 +57,22 @@ def alg_bytes_per_cell(precision: str, m_over_n: float, local: int) -> dict:
     st1 = q + common + 5 * s
     st23 = 5 * s + q + common + 5 * s
     st4 = 5 * s + q + common + q
+    if kset != "split":
+        dv, st1 = 0.0, dv + 5 * s
     return {"k_pre": pre_cells + pre_nodes, "k_dv": dv, "k_stage1": st1, "k_stage2": st23,
             "k_stage3": st23, "k_stage4": st4}
 
 
-def load_traffic(precision: str):
+def load_traffic(workload: str, precision: str, kset: str):
     """Measured HBM bytes per launch of each kernel (rocprofv3 FETCH_SIZE + WRITE_SIZE,
-    scaled by a copy kernel of known size), if a committed summary exists."""
-    path = os.path.join(REPO, "profiles", "traffic_duct_385x129.json")
+    scaled by a copy kernel of known size), if a committed summary exists for the
+    same workload, precision and kernel set."""
+    path = os.path.join(REPO, "profiles", f"traffic_{workload}.json")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
-        return json.load(fh).get(precision)
+        d = json.load(fh).get(precision)
+    return d if d and d.get("kernel_set") == kset else None
 
 
 def run_gpu(mesh, flow, precision, steps, warmup):
@@ -82,7 +92,8 @@ def run_gpu(mesh, flow, precision, steps, warmup):
         s.iterate(steps, history=False)
         ktimes = s.kernel_times()
         s.set_profiling(False)
-    return elapsed, state, l2, bad, ktimes
+        kset = s.kernel_set()
+    return elapsed, state, l2, bad, ktimes, kset
 
 
 def cpu_baseline(mesh, flow, target_s: float = 15.0):
@@ -162,9 +173,9 @@ def main():
     if args.precision:
         mk_mesh, mk_flow = gridgen.WORKLOADS[args.workload]
         mesh, flow = mk_mesh(), mk_flow()
-        el, _, _, bad, kt = run_gpu(mesh, flow, args.precision, args.steps, args.warmup)
+        el, _, _, bad, kt, ks = run_gpu(mesh, flow, args.precision, args.steps, args.warmup)
         print(f"bench: {args.precision} {mesh.neles * args.steps / el / 1e6:.1f} Mcell-iters/s "
-              f"kernels(ms) {np.round(kt, 5).tolist()}", flush=True)
+              f"kernel set {ks} kernels(ms) {np.round(kt, 5).tolist()}", flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); multi-GPU "
@@ -176,13 +187,15 @@ def main():
     print(f"bench: {args.workload} neles={n} nghosts={mesh.nghosts} steps={args.steps} "
           f"warmup={args.warmup}", file=sys.stderr, flush=True)
 
-    el64, st64, l2_64, bad64, kt64 = run_gpu(mesh, flow, "fp64", args.steps, args.warmup)
+    el64, st64, l2_64, bad64, kt64, ks64 = run_gpu(mesh, flow, "fp64", args.steps, args.warmup)
     v64 = n * args.steps / el64 / 1e6
     print(f"bench: fp64 {v64:.1f} Mcell-iters/s ({el64 * 1e3 / args.steps:.4f} ms/iter) "
-          f"kernels(ms) {np.round(kt64, 5).tolist()}", file=sys.stderr, flush=True)
+          f"kernel set {ks64} kernels(ms) {np.round(kt64, 5).tolist()}", file=sys.stderr,
+          flush=True)
     mixed = None
     if not args.no_mixed:
-        el32, st32, _, bad32, kt32 = run_gpu(mesh, flow, "fp32-mixed", args.steps, args.warmup)
+        el32, st32, _, bad32, kt32, ks32 = run_gpu(mesh, flow, "fp32-mixed", args.steps,
+                                                   args.warmup)
         v32 = n * args.steps / el32 / 1e6
         a = tdt.primitives_from_state(st32, flow)
         b = tdt.primitives_from_state(st64, flow)
@@ -191,17 +204,18 @@ def main():
         mixed = {"value": round(v32, 2), "ms_per_step": round(el32 * 1e3 / args.steps, 5),
                  "speedup_vs_fp64": round(v32 / v64, 3),
                  "rel_l2_vs_fp64_after_steps": {k: float(f"{v:.3e}") for k, v in err.items()},
-                 "kernel_ms": [round(float(x), 5) for x in kt32], "nonfinite_iter": int(bad32)}
+                 "kernel_set": ks32, "kernel_ms": [round(float(x), 5) for x in kt32],
+                 "nonfinite_iter": int(bad32)}
         print(f"bench: fp32-mixed {v32:.1f} Mcell-iters/s, rel err vs fp64 {err}", file=sys.stderr,
               flush=True)
 
     # roofline of the dominant kernel (largest average duration per launch)
-    b = alg_bytes_per_cell("fp64", mesh.nods / n, flow.local)
+    b = alg_bytes_per_cell("fp64", mesh.nods / n, flow.local, ks64)
     dom = int(np.argmax(kt64))
     name = KERNELS[dom]
     t_launch = kt64[dom] * 1e-3
     achieved = b[name] * n / t_launch / 1e9
-    traffic = load_traffic("fp64")
+    traffic = load_traffic(args.workload, "fp64", ks64)
     roof = {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": (traffic or {}).get(name),
@@ -217,10 +231,10 @@ def main():
         "warmup": args.warmup, "ms_per_step": round(el64 * 1e3 / args.steps, 5),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (seeded grid generator, cold start from uniform flow)",
-        "config": {"workload": f"{args.workload}: axisymmetric converging-diverging duct, "
+        "config": {"workload": f"{args.workload}: {DESCRIBE[args.workload]}, "
                                f"{n} cells + {mesh.nghosts} ghosts, IAXIS={flow.iaxis}, "
                                f"P0={flow.p0}, T0={flow.t0}, M={flow.amachinit}, CFL={flow.cfl}",
-                   "precision": "fp64", "nonfinite_iter": int(bad64)},
+                   "precision": "fp64", "kernel_set": ks64, "nonfinite_iter": int(bad64)},
         "fp32_mixed": mixed,
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -158,3 +158,72 @@ def test_fp32_mixed_tracks_fp64(golden):
         errs = field_errors(b.get_state(), a.get_state(), g.flow)
     assert errs["p"] < 1e-5, errs
     assert errs["rho"] < 1e-5, errs
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32-mixed"])
+@pytest.mark.parametrize("case", ["t2d_ramp", "taxi_duct", "t7_plate", "case1_axi"])
+def test_kernel_sets_identical(golden, case, precision, monkeypatch):
+    """The merged, split and quad kernel sets evaluate the same expressions in
+    the same order.  FP64: states bitwise identical, and the monitor norms too
+    except for the quad set, whose blocks hold 64 cells instead of 256 and so
+    sum the squares in another order (rounding-level differences only).
+    FP32-mixed is compiled with FMA contraction and reciprocal math, which the
+    compiler applies per kernel, so the sets agree to FP32 rounding there."""
+    import tdt
+    g = golden(case)
+    runs = {}
+    for ks in tdt.Solver.KERNEL_SETS:
+        monkeypatch.setenv("TDT_KERNELS", ks)
+        with tdt.Solver(g.mesh, g.flow, precision) as s:
+            s.cold_start()
+            l2, linf, _ = s.iterate(40)
+            assert s.kernel_set() == ks
+            runs[ks] = (s.get_state(), l2, linf)
+    base = runs["merged"]
+    for ks, (st, l2, linf) in runs.items():
+        if precision == "fp32-mixed":
+            a = tdt.primitives_from_state(st, g.flow)
+            b = tdt.primitives_from_state(base[0], g.flow)
+            speed = np.sqrt(np.linalg.norm(b["u"]) ** 2 + np.linalg.norm(b["v"]) ** 2)
+            errs = {k: np.linalg.norm(a[k] - b[k]) / max(np.linalg.norm(b[k]), 1e-300)
+                    for k in ("rho", "p", "nut")}
+            assert max(errs.values()) <= 1e-6, (ks, errs)
+            # low-Mach velocities come from small differences of momenta: scaled
+            # by the speed (v is near zero in channels), FP32 rounding shows more
+            for k in ("u", "v"):
+                errs[k] = np.linalg.norm(a[k] - b[k]) / speed
+            assert max(errs["u"], errs["v"]) <= 1e-5, (ks, errs)
+            continue
+        assert np.array_equal(st, base[0]), f"{ks}: max diff {np.max(np.abs(st - base[0]))}"
+        assert np.array_equal(linf, base[2]), ks
+        if ks == "quad":
+            rel = np.max(np.abs(l2 - base[1]) / np.maximum(np.abs(base[1]), 1e-300))
+            assert rel <= 1e-13, f"quad L2 history: relative deviation {rel:.3e}"
+        else:
+            assert np.array_equal(l2, base[1]), ks
+
+
+def test_autotune_is_transparent(golden, monkeypatch):
+    """Timing the kernel sets on the live state and restoring it leaves the
+    solve exactly where a pinned run is."""
+    import tdt
+    g = golden("case1")
+    out = {}
+    for ks in ("merged", None):
+        if ks:
+            monkeypatch.setenv("TDT_KERNELS", ks)
+        else:
+            monkeypatch.delenv("TDT_KERNELS", raising=False)
+        with tdt.Solver(g.mesh, g.flow, "fp64") as s:
+            s.cold_start()
+            assert s.kernel_set() == ""
+            l2a, _, _ = s.iterate(5)       # the first call tunes
+            assert s.kernel_set() in tdt.Solver.KERNEL_SETS
+            l2b, _, _ = s.iterate(60)
+            out[ks] = (s.get_state(), np.vstack([l2a, l2b]), s.kernel_set())
+    assert np.array_equal(out[None][0], out["merged"][0])
+    if out[None][2] == "quad":      # L2 sums in 64-cell blocks: another order
+        rel = np.abs(out[None][1] - out["merged"][1]) / np.abs(out["merged"][1])
+        assert np.max(rel) <= 1e-13
+    else:
+        assert np.array_equal(out[None][1], out["merged"][1])
