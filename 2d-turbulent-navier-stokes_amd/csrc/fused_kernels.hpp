@@ -62,7 +62,7 @@ __global__ void __launch_bounds__(kBlock) k_pre(FDev F, Consts<double> K, Consts
             double u, v, t, p;
             uvtp_from_q(c0, c1, c2, c3, u, v, t, p, K);
             double vel = sqrt(u * u + v * v);
-            double dt = K.cfl * F.dl[i] / (fabs(vel) + sqrt(K.gamma_rr * t));
+            double dt = fdiv(K.cfl * F.dl[i], fabs(vel) + sqrt(K.gamma_rr * t));
             if (K.local) F.dt[i] = dt;
             dtl = dt;
             int4 nb4 = F.nc4[i];
@@ -80,12 +80,12 @@ __global__ void __launch_bounds__(kBlock) k_pre(FDev F, Consts<double> K, Consts
             }
             double anun = fabs(pn[0] - p) + fabs(pn[1] - p) + fabs(pn[2] - p) + fabs(pn[3] - p);
             double anud = fabs(pn[0] + p) + fabs(pn[1] + p) + fabs(pn[2] + p) + fabs(pn[3] + p);
-            stv<SD>(F.anu, i, anun / anud);
+            stv<SD>(F.anu, i, fdiv(anun, anud));
             anun = fabs(cn[4][0] - c4) + fabs(cn[4][1] - c4) + fabs(cn[4][2] - c4) +
                    fabs(cn[4][3] - c4);
             anud = fabs(cn[4][0] + c4) + fabs(cn[4][1] + c4) + fabs(cn[4][2] + c4) +
                    fabs(cn[4][3] + c4);
-            stv<SD>(F.anurt, i, anun / dmax(anud, 8.0 * c0 * K.rtol));
+            stv<SD>(F.anurt, i, fdiv(anun, dmax(anud, 8.0 * c0 * K.rtol)));
             const double ci[5] = {c0, c1, c2, c3, c4};
             for (int m = 0; m < 5; ++m)
                 stv<SD>(F.d2q, (size_t)m * nt + i,
@@ -122,7 +122,7 @@ __global__ void __launch_bounds__(kBlock) k_pre(FDev F, Consts<double> K, Consts
             ghost_state<R>(s, ty, pp, uo, vo, to, po, (R)F.ycc[id], KR);
             u = s.u; vv = s.v; t = s.t; rho = s.c[0]; c5 = s.c[4];
         }
-        R rr = c5 / rho;
+        R rr = fdiv(c5, rho);
         if (j >= ov.x) au += u;
         if (j >= ov.y) av += vv;
         at += t;
@@ -131,11 +131,12 @@ __global__ void __launch_bounds__(kBlock) k_pre(FDev F, Consts<double> K, Consts
     }
     R cnt = (R)(j1 - j0);
     const int m = F.m;
-    stv<SD>(F.node, v, au / cnt);
-    stv<SD>(F.node, (size_t)m + v, av / cnt);
-    stv<SD>(F.node, (size_t)2 * m + v, at / cnt);
-    stv<SD>(F.node, (size_t)3 * m + v, arho / cnt);
-    stv<SD>(F.node, (size_t)4 * m + v, dmax(KR.rtol, ar / cnt));
+    const Recip<R> rcnt(cnt);
+    stv<SD>(F.node, v, rcnt.div(au));
+    stv<SD>(F.node, (size_t)m + v, rcnt.div(av));
+    stv<SD>(F.node, (size_t)2 * m + v, rcnt.div(at));
+    stv<SD>(F.node, (size_t)3 * m + v, rcnt.div(arho));
+    stv<SD>(F.node, (size_t)4 * m + v, dmax(KR.rtol, rcnt.div(ar)));
 }
 
 // ---------------------------------------------------------------------------------
@@ -156,7 +157,7 @@ __global__ void __launch_bounds__(kBlock) k_dv(FDev F, Consts<double> K, Consts<
     double ci[5];
     for (int q = 0; q < 5; ++q) ci[q] = Q[q * nt + i];
     R voli = (R)F.vol[i];
-    R volbydt = voli / (R)dti;
+    R volbydt = fdiv(voli, (R)dti);
     R anui = (R)ld<SD>(F.anu, i), anurti = (R)ld<SD>(F.anurt, i);
     R d2qi[5];
     for (int q = 0; q < 5; ++q) d2qi[q] = (R)ld<SD>(F.d2q, (size_t)q * nt + i);
@@ -186,7 +187,7 @@ __global__ void __launch_bounds__(kBlock) k_dv(FDev F, Consts<double> K, Consts<
     R ua, va, ta, pa;
     R rhoa = (R)ci[0];
     uvtp_from_q<R>(rhoa, (R)ci[1], (R)ci[2], (R)ci[3], ua, va, ta, pa, KR);
-    R ra = (R)ci[4] / rhoa;
+    R ra = fdiv((R)ci[4], rhoa);
     R xa = (R)F.xcc[i], ya = (R)F.ycc[i];
     Prim<R> pi;
     bool have_pi = false;
@@ -215,7 +216,7 @@ __global__ void __launch_bounds__(kBlock) k_dv(FDev F, Consts<double> K, Consts<
             ghost_state<R>(s, ty, pi, uo, vo, to, po, (R)F.ycc[c], KR);
             uc = s.u; vc = s.v; tc = s.t; rhoc = s.c[0]; c5c = s.c[4];
         }
-        R rc = c5c / rhoc;
+        R rc = fdiv(c5c, rhoc);
         ViscFace<R> o = viscous_face<R>(
             (R)F.sc[(2 * f) * n + i], (R)F.sc[(2 * f + 1) * n + i], xa, ya, (R)F.xn[bnode],
             (R)F.yn[bnode], (R)F.xcc[c], (R)F.ycc[c], (R)F.xn[dnode], (R)F.yn[dnode], ua, va, ta,
@@ -276,7 +277,7 @@ __global__ void __launch_bounds__(kBlock) k_rk(FDev F, Consts<double> K, Consts<
                 d2qi[q] = (R)ld<SD>(F.d2q, (size_t)q * nt + i);
             }
             const R anui = (R)ld<SD>(F.anu, i), anurti = (R)ld<SD>(F.anurt, i);
-            const R volbydt = (R)F.vol[i] / (R)dti;
+            const R volbydt = fdiv((R)F.vol[i], (R)dti);
             R d2[5], d4[5];
 #pragma unroll
             for (int f = 0; f < 4; ++f) {
@@ -355,11 +356,11 @@ __global__ void __launch_bounds__(kBlock) k_rk(FDev F, Consts<double> K, Consts<
                 const int m = F.m;
                 const ViscFace<R> o = viscous_face<R>(
                     sx, sy, (R)F.xcc[i], (R)F.ycc[i], (R)F.xn[bn], (R)F.yn[bn], xj, yj,
-                    (R)F.xn[dn], (R)F.yn[dn], pi.u, pi.v, pi.t, pi.c[0], pi.c[4] / pi.c[0],
+                    (R)F.xn[dn], (R)F.yn[dn], pi.u, pi.v, pi.t, pi.c[0], fdiv(pi.c[4], pi.c[0]),
                     (R)ld<SD>(F.node, bn), (R)ld<SD>(F.node, (size_t)m + bn),
                     (R)ld<SD>(F.node, (size_t)2 * m + bn), (R)ld<SD>(F.node, (size_t)3 * m + bn),
                     (R)ld<SD>(F.node, (size_t)4 * m + bn), pj.u, pj.v, pj.t, pj.c[0],
-                    pj.c[4] / pj.c[0], (R)ld<SD>(F.node, dn), (R)ld<SD>(F.node, (size_t)m + dn),
+                    fdiv(pj.c[4], pj.c[0]), (R)ld<SD>(F.node, dn), (R)ld<SD>(F.node, (size_t)m + dn),
                     (R)ld<SD>(F.node, (size_t)2 * m + dn), (R)ld<SD>(F.node, (size_t)3 * m + dn),
                     (R)ld<SD>(F.node, (size_t)4 * m + dn), KR);
                 vu = vu + o.u0 + o.u1;
@@ -383,7 +384,7 @@ __global__ void __launch_bounds__(kBlock) k_rk(FDev F, Consts<double> K, Consts<
         for (int q = 0; q < 5; ++q) r[q] = (double)(R(0.5) * acc[q]);
         r[4] = r[4] - vol * (double)s5;
         if (K.iaxis == 1) r[2] = r[2] - vol * (double)s3;
-        const double a = alpha * (dti / vol);
+        const double a = alpha * fdiv(dti, vol);
 #pragma unroll
         for (int q = 0; q < 5; ++q) cold[q] = Q[q * nt + i];
         cnew[0] = cold[0] - a * (r[0] - dd[0]);
@@ -391,7 +392,7 @@ __global__ void __launch_bounds__(kBlock) k_rk(FDev F, Consts<double> K, Consts<
         cnew[2] = cold[2] - a * (r[2] - vs[1] - dd[2]);
         cnew[3] = cold[3] - a * (r[3] - vs[2] - dd[3]);
         cnew[4] = cold[4] - a * kCflTurb * (r[4] - vs[3] - dd[4]);
-        cnew[4] = cnew[0] * dmax(K.rtol, cnew[4] / cnew[0]);
+        cnew[4] = cnew[0] * dmax(K.rtol, fdiv(cnew[4], cnew[0]));
         if (STAGE < 4) {
             void *qo = F.st[(STAGE - 1) & 1];
 #pragma unroll

@@ -10,6 +10,11 @@ namespace tdt {
 
 namespace {
 
+// 4 x (256 CUs x 4 SIMDs x 64 lanes): from here on a kernel runs several
+// rounds of wavefronts per SIMD and throughput, not the slowest wavefront,
+// sets its time
+constexpr int kBoundaryLastMinCells = 4 * 65536;
+
 std::string fmt(const char *f, long a = 0, long b = 0, long c = 0) {
     char buf[256];
     std::snprintf(buf, sizeof buf, f, a, b, c);
@@ -163,7 +168,10 @@ std::string prepare_mesh(const tdt_mesh &ms, int ordering, PreparedMesh &pm, int
     std::vector<int> new2old = ident;
     pm.ordering_used = TDT_ORDER_INPUT;
     pm.bandwidth = bw_in;
-    if (ordering == TDT_ORDER_RCM || ordering == TDT_ORDER_AUTO) {
+    const bool boundary_last = ordering == TDT_ORDER_BOUNDARY_LAST ||
+                               (ordering == TDT_ORDER_AUTO && n >= kBoundaryLastMinCells);
+    if (ordering == TDT_ORDER_RCM || ordering == TDT_ORDER_AUTO ||
+        ordering == TDT_ORDER_BOUNDARY_LAST) {
         std::vector<int> rcm = rcm_order(n, nc_int), o2n(n);
         for (int q = 0; q < n; ++q) o2n[rcm[q]] = q;
         int bw_rcm = order_bandwidth(n, nc_int, o2n);
@@ -175,6 +183,32 @@ std::string prepare_mesh(const tdt_mesh &ms, int ordering, PreparedMesh &pm, int
     } else if (ordering != TDT_ORDER_INPUT) {
         *code = TDT_ERR_ARG;
         return fmt("unknown ordering %ld", ordering);
+    }
+    if (boundary_last) {
+        // Cells with a ghost neighbour go last, grouped by which boundary type
+        // sits on which face (stable within a group).  A wavefront executes the
+        // boundary-condition code of every (face, type) that any of its lanes
+        // has: interior wavefronts then skip it all, and a boundary wavefront
+        // runs one such branch instead of a mix.  The price is scattered
+        // neighbour gathers in the boundary wavefronts, which sets the kernel
+        // time on grids too small to fill the GPU (all wavefronts then run at
+        // once and the slowest decides), so small grids keep the plain order.
+        auto signature = [&](int c) {
+            int sig = 0;
+            for (int f = 0; f < 4; ++f) {
+                int v = nc_old[(size_t)f * n + c];
+                sig = sig * 8 + (v < 0 ? ms.type[-v - 2] : 0);
+            }
+            return sig;
+        };
+        std::vector<int> sig(n);
+        for (int c = 0; c < n; ++c) sig[c] = signature(c);
+        std::stable_sort(new2old.begin(), new2old.end(),
+                         [&](int a, int b) { return sig[a] < sig[b]; });
+        std::vector<int> o2n(n);
+        for (int q = 0; q < n; ++q) o2n[new2old[q]] = q;
+        pm.bandwidth = order_bandwidth(n, nc_int, o2n);
+        pm.ordering_used = TDT_ORDER_BOUNDARY_LAST;
     }
     pm.cell_new2old = new2old;
     pm.cell_old2new.assign(n, 0);

@@ -53,6 +53,45 @@ __device__ __forceinline__ float ttanh(float a) { return tanhf(a); }
 __device__ __forceinline__ double tsqrt(double a) { return sqrt(a); }
 __device__ __forceinline__ float tsqrt(float a) { return sqrtf(a); }
 
+// Division by a reused denominator.  FP64 division on gfx950 is r = rcp(b),
+// two Newton steps r += r (1 - b r), q = a r, q += r (a - b q) (Markstein's
+// correction, correctly rounded given the refined r), wrapped in operand
+// pre-scaling (v_div_scale), post-scaling (v_div_fmas) and a special-case
+// fix-up (v_div_fixup) that change nothing unless an operand is zero, infinite,
+// NaN or denormal, the quotient is denormal or overflows, or the exponents are
+// about 768 apart.  Recip keeps the refined reciprocal of b, so each numerator
+// costs a multiply and two FMAs instead of the whole sequence; for the finite,
+// normal operands of a physical state the result has the same bits as a / b.
+// Outside that range: a / 0, inf / b and b = inf give NaN (a / b: inf or 0),
+// still non-finite where a / b is; -0 / b gives +0 for b > 0.
+// FP32 (mixed path): plain a / b, compiled there with reciprocal math.
+template <class R>
+struct Recip;
+template <>
+struct Recip<double> {
+    double b, r;
+    __device__ __forceinline__ explicit Recip(double den) : b(den) {
+        const double r0 = __builtin_amdgcn_rcp(den);
+        const double r1 = __builtin_fma(r0, __builtin_fma(-den, r0, 1.0), r0);
+        r = __builtin_fma(r1, __builtin_fma(-den, r1, 1.0), r1);
+    }
+    __device__ __forceinline__ double div(double a) const {
+        const double q = a * r;
+        return __builtin_fma(__builtin_fma(-b, q, a), r, q);
+    }
+};
+template <>
+struct Recip<float> {
+    float b;
+    __device__ __forceinline__ explicit Recip(float den) : b(den) {}
+    __device__ __forceinline__ float div(float a) const { return a / b; }
+};
+
+// a / b through Recip: same bits as a / b under the conditions above, three
+// VALU instructions shorter (FP64)
+template <class R>
+__device__ __forceinline__ R fdiv(R a, R b) { return Recip<R>(b).div(a); }
+
 // real**integer by binary exponentiation (Fortran's real**integer).
 template <class R>
 __device__ __forceinline__ R ipow(R a, int b) {
@@ -81,10 +120,11 @@ __device__ __forceinline__ void prim_from_q(Prim<R> &s, R q0, R q1, R q2, R q3, 
                                             const Consts<R> &K) {
     R rho = q0;
     s.c[0] = q0; s.c[1] = q1; s.c[2] = q2; s.c[3] = q3; s.c[4] = q4;
-    s.u = q1 / rho;
-    s.v = q2 / rho;
-    R e = q3 / rho;
-    s.t = (e - R(0.5) * (s.u * s.u + s.v * s.v)) / K.cv;
+    const Recip<R> rr(rho);
+    s.u = rr.div(q1);
+    s.v = rr.div(q2);
+    R e = rr.div(q3);
+    s.t = Recip<R>(K.cv).div(e - R(0.5) * (s.u * s.u + s.v * s.v));
     s.p = rho * K.rr * s.t;
     s.f[0] = q1;
     s.f[1] = rho * s.u * s.u + s.p;
@@ -94,7 +134,7 @@ __device__ __forceinline__ void prim_from_q(Prim<R> &s, R q0, R q1, R q2, R q3, 
     s.g[1] = s.f[2];
     s.g[2] = rho * s.v * s.v + s.p;
     s.g[3] = (q3 + s.p) * s.v;
-    R rt = q4 / rho;
+    R rt = rr.div(q4);
     s.f[4] = rho * rt * s.u;
     s.g[4] = rho * rt * s.v;
 }
@@ -104,10 +144,11 @@ template <class R>
 __device__ __forceinline__ void uvtp_from_q(R q0, R q1, R q2, R q3, R &u, R &v, R &t, R &p,
                                             const Consts<R> &K) {
     R rho = q0;
-    u = q1 / rho;
-    v = q2 / rho;
-    R e = q3 / rho;
-    t = (e - R(0.5) * (u * u + v * v)) / K.cv;
+    const Recip<R> rr(rho);
+    u = rr.div(q1);
+    v = rr.div(q2);
+    R e = rr.div(q3);
+    t = Recip<R>(K.cv).div(e - R(0.5) * (u * u + v * v));
     p = rho * K.rr * t;
 }
 
@@ -221,15 +262,15 @@ __device__ __forceinline__ ViscFace<R> viscous_face(R sx, R sy, R xa, R ya, R xb
                                                     R ra, R ub, R vb, R tb, R rhob, R rb, R uc,
                                                     R vc, R tc, R rhoc, R rc, R ud, R vd, R td,
                                                     R rhod, R rd, const Consts<R> &K) {
-    R dnr = (xa - xc) * (yb - yd) - (xb - xd) * (ya - yc);
-    R dux = ((ua - uc) * (yb - yd) - (ub - ud) * (ya - yc)) / dnr;
-    R dvx = ((va - vc) * (yb - yd) - (vb - vd) * (ya - yc)) / dnr;
-    R dtx = ((ta - tc) * (yb - yd) - (tb - td) * (ya - yc)) / dnr;
-    R drx = ((ra - rc) * (yb - yd) - (rb - rd) * (ya - yc)) / dnr;
-    R duy = (-((ua - uc) * (xb - xd)) + (ub - ud) * (xa - xc)) / dnr;
-    R dvy = (-((va - vc) * (xb - xd)) + (vb - vd) * (xa - xc)) / dnr;
-    R dty = (-((ta - tc) * (xb - xd)) + (tb - td) * (xa - xc)) / dnr;
-    R dry = (-((ra - rc) * (xb - xd)) + (rb - rd) * (xa - xc)) / dnr;
+    const Recip<R> dnr((xa - xc) * (yb - yd) - (xb - xd) * (ya - yc));
+    R dux = dnr.div((ua - uc) * (yb - yd) - (ub - ud) * (ya - yc));
+    R dvx = dnr.div((va - vc) * (yb - yd) - (vb - vd) * (ya - yc));
+    R dtx = dnr.div((ta - tc) * (yb - yd) - (tb - td) * (ya - yc));
+    R drx = dnr.div((ra - rc) * (yb - yd) - (rb - rd) * (ya - yc));
+    R duy = dnr.div(-((ua - uc) * (xb - xd)) + (ub - ud) * (xa - xc));
+    R dvy = dnr.div(-((va - vc) * (xb - xd)) + (vb - vd) * (xa - xc));
+    R dty = dnr.div(-((ta - tc) * (xb - xd)) + (tb - td) * (xa - xc));
+    R dry = dnr.div(-((ra - rc) * (xb - xd)) + (rb - rd) * (xa - xc));
     R uavg = R(0.25) * (ua + ub + uc + ud);
     R vavg = R(0.25) * (va + vb + vc + vd);
     R tavg = R(0.25) * (ta + tb + tc + td);
@@ -238,17 +279,17 @@ __device__ __forceinline__ ViscFace<R> viscous_face(R sx, R sy, R xa, R ya, R xb
     R divvel;
     if (K.iaxis == 1) {
         R yavg = R(0.5) * (yb + yd);
-        R vbyy = (yavg < R(kYaxisThresh)) ? dvy : vavg / yavg;
+        R vbyy = (yavg < R(kYaxisThresh)) ? dvy : fdiv(vavg, yavg);
         divvel = dux + dvy + vbyy;
     } else {
         divvel = dux + dvy;
     }
     R amu = K.suth_sqrt_amol * tpow(tavg, R(0.6));
-    R ak = amu * K.cp / K.prandtl;
-    R chi = rhoavg * ravg / (amu * R(kCmu));
-    R fmu = ttanh(R(kAlphaTurb) * chi * chi) / ttanh(R(kBetaTurb) * chi * chi);
+    R ak = fdiv(amu * K.cp, K.prandtl);
+    R chi = fdiv(rhoavg * ravg, amu * R(kCmu));
+    R fmu = fdiv(ttanh(R(kAlphaTurb) * chi * chi), ttanh(R(kBetaTurb) * chi * chi));
     R amut = rhoavg * ravg * fmu;
-    R akt = amut * K.cp / K.prandtlt;
+    R akt = fdiv(amut * K.cp, K.prandtlt);
     R amueff = amu + amut;
     R akeff = ak + akt;
     R txx = amueff * (dux + dux - R(kDiv23) * divvel);
@@ -282,7 +323,7 @@ struct SrcNbr {
 
 template <class R>
 __device__ __forceinline__ SrcNbr<R> src_nbr(R x, R y, R u, R v, R c1, R c5) {
-    return SrcNbr<R>{x, y, tsqrt(u * u + v * v), c5 / c1, u, v};
+    return SrcNbr<R>{x, y, tsqrt(u * u + v * v), Recip<R>(c1).div(c5), u, v};
 }
 
 template <class R>
@@ -296,40 +337,42 @@ __device__ __forceinline__ void src_pair(SrcNbr<R> a, SrcNbr<R> c, R &dx, R &dy,
 template <class R>
 __device__ __forceinline__ R source_terms_d(const SrcDiff<R> &d, R rho, R c5, R t, R p, R v,
                                             R ycc, R &src3, const Consts<R> &K) {
-    R dnr = d.xac * d.ybd - d.xbd * d.yac;
-    R dqx = (d.qac * d.ybd - d.qbd * d.yac) / dnr;
-    R drx = (d.rac * d.ybd - d.rbd * d.yac) / dnr;
-    R dux = (d.uac * d.ybd - d.ubd * d.yac) / dnr;
-    R dvx = (d.vac * d.ybd - d.vbd * d.yac) / dnr;
-    R dqy = (-(d.qac * d.xbd) + d.qbd * d.xac) / dnr;
-    R dry = (-(d.rac * d.xbd) + d.rbd * d.xac) / dnr;
-    R duy = (-(d.uac * d.xbd) + d.ubd * d.xac) / dnr;
-    R dvy = (-(d.vac * d.xbd) + d.vbd * d.xac) / dnr;
-    R ri = c5 / rho;
+    const Recip<R> dnr(d.xac * d.ybd - d.xbd * d.yac);
+    R dqx = dnr.div(d.qac * d.ybd - d.qbd * d.yac);
+    R drx = dnr.div(d.rac * d.ybd - d.rbd * d.yac);
+    R dux = dnr.div(d.uac * d.ybd - d.ubd * d.yac);
+    R dvx = dnr.div(d.vac * d.ybd - d.vbd * d.yac);
+    R dqy = dnr.div(-(d.qac * d.xbd) + d.qbd * d.xac);
+    R dry = dnr.div(-(d.rac * d.xbd) + d.rbd * d.xac);
+    R duy = dnr.div(-(d.uac * d.xbd) + d.ubd * d.xac);
+    R dvy = dnr.div(-(d.vac * d.xbd) + d.vbd * d.xac);
+    const Recip<R> rrho(rho);
+    R ri = rrho.div(c5);
     R amu = K.suth_sqrt_amol * tpow(t, R(0.6));
-    R chi = rho * ri / (R(kCmu) * amu);
-    R fmu = ttanh(R(kAlphaTurb) * chi * chi) / ttanh(R(kBetaTurb) * chi * chi);
-    R fr2 = ttanh(tsqrt(chi)) / ttanh(chi * K.sqcmu);
+    R chi = fdiv(rho * ri, R(kCmu) * amu);
+    R fmu = fdiv(ttanh(R(kAlphaTurb) * chi * chi), ttanh(R(kBetaTurb) * chi * chi));
+    R fr2 = fdiv(ttanh(tsqrt(chi)), ttanh(chi * K.sqcmu));
     R amut = fmu * rho * ri;
     R phi = dqx * drx + dqy * dry;
     R dee = R(0);
     if (phi > R(0)) dee = drx * drx + dry * dry;
-    R ratio = ipow(amut / dmax(amut, R(kAmuThresh) * amu), K.npseudo);
+    R ratio = ipow(fdiv(amut, dmax(amut, R(kAmuThresh) * amu)), K.npseudo);
     R s5;
     if (K.iaxis == 1) {
-        R vbyy = (ycc < R(kYaxisThresh)) ? dvy : v / ycc;
+        const Recip<R> rycc(ycc);
+        R vbyy = (ycc < R(kYaxisThresh)) ? dvy : rycc.div(v);
         R divvel = dux + dvy + vbyy;
-        R pk = amut / rho *
+        R pk = rrho.div(amut) *
                (dux * dux + dvy * dvy + vbyy * vbyy + dux * dux + dvy * dvy + vbyy * vbyy +
                 (duy + dvx) * (duy + dvx) - R(kDiv23) * divvel * divvel);
         pk = dmax(pk, R(0));
         s5 = (rho * (R(kConst1) - R(kConst2) * fr2) * tsqrt(ri * pk) - rho * R(kConst3) * dee) *
              ratio;
         R ttt = (amu + amut) * (vbyy + vbyy - R(kDiv23) * divvel);
-        src3 = (p - ttt) / ycc;
+        src3 = rycc.div(p - ttt);
     } else {
         R divvel = dux + dvy;
-        R pk = amut / rho *
+        R pk = rrho.div(amut) *
                (dux * dux + dvy * dvy + dux * dux + dvy * dvy + (duy + dvx) * (duy + dvx) -
                 R(kDiv23) * divvel * divvel);
         pk = dmax(pk, R(0));
@@ -357,7 +400,7 @@ __device__ __forceinline__ void jst_coeffs(R si, R sn, R volbydt, R vol_n, R dt_
                                            R &d4n) {
     R e2n = R(kC02) * dmax(si, sn);
     R e4n = dmax(R(0), R(kC04) - e2n);
-    R d24n = R(0.5) * (volbydt + vol_n / dt_n);
+    R d24n = R(0.5) * (volbydt + fdiv(vol_n, dt_n));
     d2n = e2n * d24n;
     d4n = e4n * d24n;
 }

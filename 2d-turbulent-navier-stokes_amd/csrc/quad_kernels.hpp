@@ -34,7 +34,7 @@ __global__ void __launch_bounds__(kBlock) kq_pre(FDev F, Consts<double> K, Const
         double u, v, t, p;
         uvtp_from_q(c0, c1, c2, c3, u, v, t, p, K);
         double vel = sqrt(u * u + v * v);
-        double dt = K.cfl * F.dl[i] / (fabs(vel) + sqrt(K.gamma_rr * t));
+        double dt = fdiv(K.cfl * F.dl[i], fabs(vel) + sqrt(K.gamma_rr * t));
         if (act && lane == 0 && K.local) F.dt[i] = dt;
         // the neighbour across this lane's face; a ghost holds the parent's copy
         // in DISSIPATION (solver_state.f90:77-88), i.e. this cell
@@ -57,9 +57,9 @@ __global__ void __launch_bounds__(kBlock) kq_pre(FDev F, Consts<double> K, Const
             stv<SD>(F.d2q, (size_t)lane * nt + i, sel4(lane, d2q[0], d2q[1], d2q[2], d2q[3]));
             if (lane == 0) {
                 stv<SD>(F.d2q, (size_t)4 * nt + i, d2q[4]);
-                stv<SD>(F.anu, i, anun / anud);
+                stv<SD>(F.anu, i, fdiv(anun, anud));
             } else if (lane == 1) {
-                stv<SD>(F.anurt, i, rtn / dmax(rtd, 8.0 * c0 * K.rtol));
+                stv<SD>(F.anurt, i, fdiv(rtn, dmax(rtd, 8.0 * c0 * K.rtol)));
             }
         }
         if (!K.local) block_atomic_min_pos(act ? dt : 100000.0, reinterpret_cast<double *>(F.dtslot));
@@ -96,7 +96,7 @@ __global__ void __launch_bounds__(kBlock) kq_pre(FDev F, Consts<double> K, Const
             ghost_state<R>(s, ty, pp, uo, vo, to, po, (R)F.ycc[id], KR);
             uu = s.u; vv = s.v; tt = s.t; rho = s.c[0]; c5 = s.c[4];
         }
-        const R rr = c5 / rho;
+        const R rr = fdiv(c5, rho);
 #define TDT_NODE_ACC(K_)                                                           \
         {                                                                          \
             const int pos = base + K_;                                             \
@@ -116,9 +116,10 @@ __global__ void __launch_bounds__(kBlock) kq_pre(FDev F, Consts<double> K, Const
     if (actv) {
         const R cnt = (R)(j1 - j0);
         const int m = F.m;
-        const R val = sel4(lane, au / cnt, av / cnt, at / cnt, arho / cnt);
+        const Recip<R> rcnt(cnt);
+        const R val = rcnt.div(sel4(lane, au, av, at, arho));
         stv<SD>(F.node, (size_t)lane * m + v, val);
-        if (lane == 0) stv<SD>(F.node, (size_t)4 * m + v, dmax(KR.rtol, ar / cnt));
+        if (lane == 0) stv<SD>(F.node, (size_t)4 * m + v, dmax(KR.rtol, rcnt.div(ar)));
     }
 }
 
@@ -163,7 +164,7 @@ __global__ void __launch_bounds__(kBlock) kq_stage(FDev F, Consts<double> K, Con
         // dissipation face terms; a ghost neighbour is a copy of this cell
         const double dti = K.local ? F.dt[i] : __longlong_as_double((long long)*F.dtslot);
         const R voli = (R)F.vol[i];
-        const R volbydt = voli / (R)dti;
+        const R volbydt = fdiv(voli, (R)dti);
         const R dtj = K.local ? (R)F.dt[jj] : (R)dti;
         const R volj = (R)F.vol[jj];
         R a2, a4, b2, b4;
@@ -181,7 +182,7 @@ __global__ void __launch_bounds__(kBlock) kq_stage(FDev F, Consts<double> K, Con
         const int bnode = sel4(lane, nd4.x, nd4.y, nd4.z, nd4.w);
         const int dnode = sel4(lane, nd4.y, nd4.z, nd4.w, nd4.x);
         const int m = F.m;
-        const R ra = pi.c[4] / pi.c[0], rc = pj.c[4] / pj.c[0];
+        const R ra = fdiv(pi.c[4], pi.c[0]), rc = fdiv(pj.c[4], pj.c[0]);
         ViscFace<R> o = viscous_face<R>(
             sx, sy, (R)F.xcc[i], (R)F.ycc[i], (R)F.xn[bnode], (R)F.yn[bnode], xj, yj,
             (R)F.xn[dnode], (R)F.yn[dnode], pi.u, pi.v, pi.t, pi.c[0], ra,
@@ -201,47 +202,50 @@ __global__ void __launch_bounds__(kBlock) kq_stage(FDev F, Consts<double> K, Con
     // four neighbour centroids; lane f forms the x/y gradient of quantity f
     const R xa = qget<0>(xj), xb = qget<1>(xj), xc = qget<2>(xj), xd = qget<3>(xj);
     const R ya = qget<0>(yj), yb = qget<1>(yj), yc = qget<2>(yj), yd = qget<3>(yj);
-    const R rj = pj.c[4] / pj.c[0];
+    const R rj = fdiv(pj.c[4], pj.c[0]);
     const R qj = tsqrt(pj.u * pj.u + pj.v * pj.v);
     const R wa = sel4(lane, qget<0>(qj), qget<0>(rj), qget<0>(pj.u), qget<0>(pj.v));
     const R wb = sel4(lane, qget<1>(qj), qget<1>(rj), qget<1>(pj.u), qget<1>(pj.v));
     const R wc = sel4(lane, qget<2>(qj), qget<2>(rj), qget<2>(pj.u), qget<2>(pj.v));
     const R wd = sel4(lane, qget<3>(qj), qget<3>(rj), qget<3>(pj.u), qget<3>(pj.v));
     const R dnr = (xa - xc) * (yb - yd) - (xb - xd) * (ya - yc);
-    const R gx = ((wa - wc) * (yb - yd) - (wb - wd) * (ya - yc)) / dnr;
-    const R gy = (-((wa - wc) * (xb - xd)) + (wb - wd) * (xa - xc)) / dnr;
+    const Recip<R> rdnr(dnr);
+    const R gx = rdnr.div((wa - wc) * (yb - yd) - (wb - wd) * (ya - yc));
+    const R gy = rdnr.div(-((wa - wc) * (xb - xd)) + (wb - wd) * (xa - xc));
     const R dqx = qget<0>(gx), dqy = qget<0>(gy), drx = qget<1>(gx), dry = qget<1>(gy);
     const R dux = qget<2>(gx), duy = qget<2>(gy), dvx = qget<3>(gx), dvy = qget<3>(gy);
     const R rho = pi.c[0];
-    const R ri = pi.c[4] / rho;
+    const Recip<R> rrho(rho);
+    const R ri = rrho.div(pi.c[4]);
     const R amu = KR.suth_sqrt_amol * tpow(pi.t, R(0.6));
-    const R chi = rho * ri / (R(kCmu) * amu);
+    const R chi = fdiv(rho * ri, R(kCmu) * amu);
     // the four tanh of FMU and FR2, one per lane
     const R targ = sel4(lane, R(kAlphaTurb) * chi * chi, R(kBetaTurb) * chi * chi, tsqrt(chi),
                         chi * KR.sqcmu);
     const R th = ttanh(targ);
-    const R fmu = qget<0>(th) / qget<1>(th);
-    const R fr2 = qget<2>(th) / qget<3>(th);
+    const R fmu = fdiv(qget<0>(th), qget<1>(th));
+    const R fr2 = fdiv(qget<2>(th), qget<3>(th));
     const R amut = fmu * rho * ri;
     const R phi = dqx * drx + dqy * dry;
     R dee = R(0);
     if (phi > R(0)) dee = drx * drx + dry * dry;
-    const R ratio = ipow(amut / dmax(amut, R(kAmuThresh) * amu), KR.npseudo);
+    const R ratio = ipow(fdiv(amut, dmax(amut, R(kAmuThresh) * amu)), KR.npseudo);
     R s5, s3 = R(0);
     if (KR.iaxis == 1) {
         const R ycci = (R)F.ycc[i];
-        const R vbyy = (ycci < R(kYaxisThresh)) ? dvy : pi.v / ycci;
+        const Recip<R> rycc(ycci);
+        const R vbyy = (ycci < R(kYaxisThresh)) ? dvy : rycc.div(pi.v);
         const R divvel = dux + dvy + vbyy;
-        R pk = amut / rho *
+        R pk = rrho.div(amut) *
                (dux * dux + dvy * dvy + vbyy * vbyy + dux * dux + dvy * dvy + vbyy * vbyy +
                 (duy + dvx) * (duy + dvx) - R(kDiv23) * divvel * divvel);
         pk = dmax(pk, R(0));
         s5 = (rho * (R(kConst1) - R(kConst2) * fr2) * tsqrt(ri * pk) - rho * R(kConst3) * dee) * ratio;
         const R ttt = (amu + amut) * (vbyy + vbyy - R(kDiv23) * divvel);
-        s3 = (pi.p - ttt) / ycci;
+        s3 = rycc.div(pi.p - ttt);
     } else {
         const R divvel = dux + dvy;
-        R pk = amut / rho *
+        R pk = rrho.div(amut) *
                (dux * dux + dvy * dvy + dux * dux + dvy * dvy + (duy + dvx) * (duy + dvx) -
                 R(kDiv23) * divvel * divvel);
         pk = dmax(pk, R(0));
@@ -263,7 +267,7 @@ __global__ void __launch_bounds__(kBlock) kq_stage(FDev F, Consts<double> K, Con
     } else {
         dti = F.dt[i];
     }
-    const double a = alpha * (dti / vol);
+    const double a = alpha * fdiv(dti, vol);
     const int e = lane;
     const double coldE = Q[e * nt + i];
     double dE, visE;
@@ -282,7 +286,7 @@ __global__ void __launch_bounds__(kBlock) kq_stage(FDev F, Consts<double> K, Con
         const double d4 = STAGE == 1 ? dcell[4] : ld<SD>(F.dd, (size_t)4 * n + i);
         const double v4 = STAGE == 1 ? vcell[3] : ld<SD>(F.vis, (size_t)3 * n + i);
         c4 = cold4 - a * kCflTurb * (rr[4] - v4 - d4);
-        c4 = cE * dmax(K.rtol, c4 / cE);
+        c4 = cE * dmax(K.rtol, fdiv(c4, cE));
     }
     if (act) {
         if (STAGE == 1) {

@@ -33,6 +33,21 @@ constexpr double kLn2Lo = -0x1.718432a1b0e26p-35;
 constexpr double kInvLn2 = 1.4426950408889634;
 constexpr double kSqrtHalf = 0.7071067811865476;
 
+// a / b.  On the device: the refined-reciprocal sequence of the compiler's
+// FP64 division without its scaling and special-case steps (not needed for the
+// normal, finite operands below); same bits as a / b there.
+TDT_HD inline double quot(double a, double b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double r0 = __builtin_amdgcn_rcp(b);
+    const double r1 = __builtin_fma(r0, __builtin_fma(-b, r0, 1.0), r0);
+    const double r = __builtin_fma(r1, __builtin_fma(-b, r1, 1.0), r1);
+    const double q = a * r;
+    return __builtin_fma(__builtin_fma(-b, q, a), r, q);
+#else
+    return a / b;
+#endif
+}
+
 // e^r - 1 for |r| <= ln2/2: r (1 + r/2! + r^2/3! + ... + r^13/14!)
 TDT_HD inline double expm1_reduced(double r) {
     double p = 1.1470745597729725e-11;          // 1/14!
@@ -81,7 +96,7 @@ TDT_HD inline double tanh(double x) {
     double y = 2.0 * std::fabs(x);
     y = y > 40.0 ? 40.0 : y;        // tanh(20) rounds to 1; NaN passes through
     const double e = expm1_pos(y);
-    return std::copysign(e / (e + 2.0), x);
+    return std::copysign(quot(e, e + 2.0), x);
 }
 
 // natural log of a positive, finite, normal x
@@ -92,7 +107,7 @@ TDT_HD inline double log_pos(double x) {
     m = low ? m + m : m;
     const double e = (double)(low ? ex - 1 : ex);
     const double f = m - 1.0;        // exact
-    const double s = f / (m + 1.0);
+    const double s = quot(f, m + 1.0);
     const double z = s * s;
     double p = 0.047619047619047616;             // 1/21
     p = std::fma(p, z, 0.05263157894736842);     // 1/19

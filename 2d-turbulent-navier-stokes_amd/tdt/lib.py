@@ -26,7 +26,7 @@ HEADER_PATH = os.path.join(os.path.dirname(PKG_DIR), "include", "tdt.h")
 TDT_OK, TDT_ERR_ARG, TDT_ERR_MESH, TDT_ERR_DEVICE, TDT_ERR_NONFINITE, TDT_ERR_UNSUPPORTED = (
     0, -1, -2, -3, -4, -5)
 PRECISION = {"fp64": 0, "fp32-mixed": 1}
-ORDERING = {"input": 0, "rcm": 1, "auto": 2}
+ORDERING = {"input": 0, "rcm": 1, "auto": 2, "boundary-last": 3}
 ERROR_NAMES = {-1: "TDT_ERR_ARG", -2: "TDT_ERR_MESH", -3: "TDT_ERR_DEVICE",
                -4: "TDT_ERR_NONFINITE", -5: "TDT_ERR_UNSUPPORTED"}
 

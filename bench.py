@@ -75,8 +75,8 @@ def load_traffic(workload: str, precision: str, kset: str):
     return d if d and d.get("kernel_set") == kset else None
 
 
-def run_gpu(mesh, flow, precision, steps, warmup):
-    with tdt.Solver(mesh, flow, precision) as s:
+def run_gpu(mesh, flow, precision, steps, warmup, ordering="auto"):
+    with tdt.Solver(mesh, flow, precision, ordering) as s:
         s.cold_start()
         if warmup:
             s.iterate(warmup, history=False)
@@ -164,6 +164,8 @@ def main():
     ap.add_argument("--no-mixed", action="store_true", help="skip the FP32-mixed run")
     ap.add_argument("--precision", default=None, choices=["fp64", "fp32-mixed"],
                     help="profile one precision only (prints no JSON line)")
+    ap.add_argument("--ordering", default="auto",
+                    choices=["auto", "rcm", "input", "boundary-last"], help="cell numbering")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="run a plain device copy of MIB MiB (HBM-counter calibration) and exit")
     args = ap.parse_args()
@@ -173,7 +175,8 @@ def main():
     if args.precision:
         mk_mesh, mk_flow = gridgen.WORKLOADS[args.workload]
         mesh, flow = mk_mesh(), mk_flow()
-        el, _, _, bad, kt, ks = run_gpu(mesh, flow, args.precision, args.steps, args.warmup)
+        el, _, _, bad, kt, ks = run_gpu(mesh, flow, args.precision, args.steps, args.warmup,
+                                        args.ordering)
         print(f"bench: {args.precision} {mesh.neles * args.steps / el / 1e6:.1f} Mcell-iters/s "
               f"kernel set {ks} kernels(ms) {np.round(kt, 5).tolist()}", flush=True)
         return
@@ -187,7 +190,8 @@ def main():
     print(f"bench: {args.workload} neles={n} nghosts={mesh.nghosts} steps={args.steps} "
           f"warmup={args.warmup}", file=sys.stderr, flush=True)
 
-    el64, st64, l2_64, bad64, kt64, ks64 = run_gpu(mesh, flow, "fp64", args.steps, args.warmup)
+    el64, st64, l2_64, bad64, kt64, ks64 = run_gpu(mesh, flow, "fp64", args.steps, args.warmup,
+                                                   args.ordering)
     v64 = n * args.steps / el64 / 1e6
     print(f"bench: fp64 {v64:.1f} Mcell-iters/s ({el64 * 1e3 / args.steps:.4f} ms/iter) "
           f"kernel set {ks64} kernels(ms) {np.round(kt64, 5).tolist()}", file=sys.stderr,
@@ -195,7 +199,7 @@ def main():
     mixed = None
     if not args.no_mixed:
         el32, st32, _, bad32, kt32, ks32 = run_gpu(mesh, flow, "fp32-mixed", args.steps,
-                                                   args.warmup)
+                                                   args.warmup, args.ordering)
         v32 = n * args.steps / el32 / 1e6
         a = tdt.primitives_from_state(st32, flow)
         b = tdt.primitives_from_state(st64, flow)

@@ -44,7 +44,12 @@ extern "C" {
 
 #define TDT_ORDER_INPUT 0 /* keep grid.in cell order                       */
 #define TDT_ORDER_RCM 1   /* reverse Cuthill-McKee                         */
-#define TDT_ORDER_AUTO 2  /* whichever of the two has the smaller bandwidth */
+#define TDT_ORDER_AUTO 2  /* TDT_ORDER_BOUNDARY_LAST on grids of 262144 cells
+                           * or more, else whichever of input and RCM has
+                           * the smaller bandwidth                          */
+#define TDT_ORDER_BOUNDARY_LAST 3 /* the smaller-bandwidth order, then cells
+                           * with a ghost neighbour moved last, grouped by
+                           * the boundary type on each face                */
 
 typedef struct tdt_ctx tdt_ctx;
 
