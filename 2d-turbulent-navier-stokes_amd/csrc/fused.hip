@@ -105,6 +105,53 @@ int fused_setup(tdt_ctx *c) {
         B->variant = ks == "merged" ? kVariantMerged : ks == "split" ? kVariantSplit : kVariantQuad;
         B->tuned = true;
     }
+    // fused-only neighbour and node-gather tables (built from the prepared mesh)
+    const PreparedMesh &pm = c->pm;
+    if (D.nt > kIdMask) {
+        c->fused = B;
+        fused_free(c);
+        c->err = "fused path: more than 2^28 cells and ghosts";
+        return TDT_ERR_UNSUPPORTED;
+    }
+    std::vector<int> nct(pm.nc4);
+    for (int &v : nct)
+        if (v >= D.n) v |= pm.gtype[v - D.n] << kTypeShift;
+    std::vector<int> ncon((size_t)2 * kNodeWidth * D.m, 0), nrel(D.m, 0);
+    for (int v = 0; v < D.m; ++v) {
+        const int j0 = pm.nptr[v], cnt = pm.nptr[v + 1] - j0;
+        int *e = &ncon[(size_t)2 * kNodeWidth * v];
+        for (int k = 0; k < kNodeWidth && k < cnt; ++k) {
+            const int id = pm.nlist[j0 + k];
+            int a = id, src = id;
+            if (id >= D.n) {
+                const int g = id - D.n, ty = pm.gtype[g];
+                src = pm.gparent[g];
+                a = (ty == 3 ? pm.gopp[g] : id) | ty << kTypeShift;
+            }
+            e[2 * k] = a;
+            e[2 * k + 1] = src;
+        }
+        for (int k = cnt; k < kNodeWidth && cnt > 0; ++k) {   // padding, unused
+            e[2 * k] = e[0];
+            e[2 * k + 1] = e[1];
+        }
+        // sum starts relative to the node's first contributor (0..cnt)
+        const int su = pm.novr[4 * v] - j0, sv = pm.novr[4 * v + 1] - j0,
+                  sr = pm.novr[4 * v + 2] - j0;
+        nrel[v] = cnt <= kNodeWidth
+                      ? cnt | su << 4 | sv << 8 | sr << 12 | (pm.novr[4 * v + 3] ? 1 << 16 : 0)
+                      : 15;   // more contributors than the records hold: list path
+    }
+    int *d_nct = dalloc<int>(*B, nct.size());
+    int *d_ncon = dalloc<int>(*B, ncon.size());
+    int *d_nrel = dalloc<int>(*B, nrel.size());
+    ok = ok && d_nct && d_ncon && d_nrel &&
+         hipMemcpy(d_nct, nct.data(), nct.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(d_ncon, ncon.data(), ncon.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(d_nrel, nrel.data(), nrel.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess;
+    F.nct = reinterpret_cast<const int4 *>(d_nct);
+    F.ncon = reinterpret_cast<const int2 *>(d_ncon);
+    F.nrel = d_nrel;
     B->qbak = dalloc<double>(*B, (size_t)5 * D.nt);
     F.dtslot = dalloc<unsigned long long>(*B, 1);
     F.ctr = dalloc<int>(*B, 4);

@@ -16,10 +16,23 @@
 
 namespace tdt {
 
+// Neighbour tables of the fused kernels carry a ghost's boundary type in the
+// top bits of its id, so a face's type needs no second dependent load.
+constexpr int kTypeShift = 28;
+constexpr int kIdMask = (1 << kTypeShift) - 1;
+// Node gathers of up to this many contributors use fixed-width records.
+constexpr int kNodeWidth = 4;
+
 struct FDev {
     int n, g, m, nt;
     int remap;                      // XCD-aware block order
     const int4 *nc4, *nod4;
+    const int4 *nct;                // [n] nc4 with ghost ids | type << kTypeShift
+    const int2 *ncon;               // [m][kNodeWidth] contributors {a, source cell}: a =
+                                    //   cell id, ghost id | type << 28, or for a
+                                    //   type-3 ghost its opposite cell | 3 << 28
+    const int *nrel;                // [m] count | U, V, nu~ sum starts << 4, 8, 12 |
+                                    //   nu~ floor flag << 16
     const double *xn, *yn;
     const int *gparent, *gtype, *gopp;
     const int *nptr, *nlist;

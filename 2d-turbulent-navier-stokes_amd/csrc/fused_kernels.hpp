@@ -44,13 +44,56 @@ __device__ __forceinline__ void fold_monitor(const FDev &F, int nblk) {
     }
 }
 
+// Node gather through the contributor list, for nodes with more than
+// kNodeWidth contributors (viscous.f90:77-124 in the reference's order).
+template <class R>
+__device__ __forceinline__ void node_gather_list(const FDev &F, const Consts<R> &KR, int v, R &au,
+                                                 R &av, R &at, R &arho, R &ar) {
+    const int nt = F.nt;
+    const double *Q = F.qn;
+    int4 ov = F.novr[v];
+    int j0 = F.nptr[v], j1 = F.nptr[v + 1];
+    for (int j = j0; j < j1; ++j) {
+        int id = F.nlist[j];
+        R u, vv, t, rho, c5;
+        if (id < F.n) {
+            R p;
+            rho = (R)Q[id];
+            uvtp_from_q<R>(rho, (R)Q[nt + id], (R)Q[2 * nt + id], (R)Q[3 * nt + id], u, vv, t, p, KR);
+            c5 = (R)Q[4 * nt + id];
+        } else {
+            int k = id - F.n, par = F.gparent[k], opp = F.gopp[k];
+            Prim<R> pp, s;
+            prim_from_q<R>(pp, (R)Q[par], (R)Q[nt + par], (R)Q[2 * nt + par], (R)Q[3 * nt + par],
+                           (R)Q[4 * nt + par], KR);
+            R uo = R(0), vo = R(0), to = R(0), po = R(0);
+            int ty = F.gtype[k];
+            if (ty == 3)
+                uvtp_from_q<R>((R)Q[opp], (R)Q[nt + opp], (R)Q[2 * nt + opp], (R)Q[3 * nt + opp],
+                               uo, vo, to, po, KR);
+            ghost_state<R>(s, ty, pp, uo, vo, to, po, (R)F.ycc[id], KR);
+            u = s.u; vv = s.v; t = s.t; rho = s.c[0]; c5 = s.c[4];
+        }
+        R rr = fdiv(c5, rho);
+        if (j >= ov.x) au += u;
+        if (j >= ov.y) av += vv;
+        at += t;
+        arho += rho;
+        if (j >= ov.z) ar += rr;
+    }
+}
+
 // ---------------------------------------------------------------------------------
-// k_pre: cells [0, nb_cells) blocks, nodes after.
+// k_pre: cells [0, nb_cells) blocks, nodes after, and a last block of its own
+// that folds the previous iteration's monitor partials (off every cell's path).
 template <class SD, class R>
 __global__ void __launch_bounds__(kBlock) k_pre(FDev F, Consts<double> K, Consts<R> KR,
                                                 int nb_cells) {
     int b = blockIdx.x;
-    if (b == 0) fold_monitor(F, nb_cells);
+    if (b == (int)gridDim.x - 1) {
+        fold_monitor(F, nb_cells);
+        return;
+    }
     if (b < nb_cells) {
         int i = b * kBlock + threadIdx.x;
         double dtl = 100000.0;
@@ -96,42 +139,61 @@ __global__ void __launch_bounds__(kBlock) k_pre(FDev F, Consts<double> K, Consts
     }
     int v = (b - nb_cells) * kBlock + threadIdx.x;
     if (v >= F.m) return;
-    const int nt = F.nt;
-    const double *Q = F.qn;
-    int4 ov = F.novr[v];
-    int j0 = F.nptr[v], j1 = F.nptr[v + 1];
-    R au = R(0), av = R(0), at = R(0), arho = R(0), ar = ov.w ? KR.rtol_rtred : R(0);
-    for (int j = j0; j < j1; ++j) {
-        int id = F.nlist[j];
-        R u, vv, t, rho, c5;
-        if (id < F.n) {
-            R p;
-            rho = (R)Q[id];
-            uvtp_from_q<R>(rho, (R)Q[nt + id], (R)Q[2 * nt + id], (R)Q[3 * nt + id], u, vv, t, p, KR);
-            c5 = (R)Q[4 * nt + id];
-        } else {
-            int k = id - F.n, par = F.gparent[k], opp = F.gopp[k];
-            Prim<R> pp, s;
-            prim_from_q<R>(pp, (R)Q[par], (R)Q[nt + par], (R)Q[2 * nt + par], (R)Q[3 * nt + par],
-                           (R)Q[4 * nt + par], KR);
-            R uo = R(0), vo = R(0), to = R(0), po = R(0);
-            int ty = F.gtype[k];
-            if (ty == 3)
-                uvtp_from_q<R>((R)Q[opp], (R)Q[nt + opp], (R)Q[2 * nt + opp], (R)Q[3 * nt + opp],
-                               uo, vo, to, po, KR);
-            ghost_state<R>(s, ty, pp, uo, vo, to, po, (R)F.ycc[id], KR);
-            u = s.u; vv = s.v; t = s.t; rho = s.c[0]; c5 = s.c[4];
+    const int hr = F.nrel[v];
+    const int cnt = hr & 15;
+    R au = R(0), av = R(0), at = R(0), arho = R(0), ar = R(0);
+    if (cnt <= kNodeWidth) {
+        // fixed-width records: the contributing cells' states are requested
+        // before any is used (a ghost's parent follows in its own branch)
+        const int nt = F.nt, n = F.n;
+        const double *Q = F.qn;
+        const int su = (hr >> 4) & 15, sv = (hr >> 8) & 15, sr = (hr >> 12) & 15;
+        ar = (hr >> 16) ? KR.rtol_rtred : R(0);
+        int2 e[kNodeWidth];
+        R q[kNodeWidth][5];
+#pragma unroll
+        for (int k = 0; k < kNodeWidth; ++k) e[k] = F.ncon[(size_t)v * kNodeWidth + k];
+#pragma unroll
+        for (int k = 0; k < kNodeWidth; ++k)
+#pragma unroll
+            for (int c = 0; c < 5; ++c) q[k][c] = (R)Q[(size_t)c * nt + e[k].y];
+#pragma unroll
+        for (int k = 0; k < kNodeWidth; ++k) {
+            if (k >= cnt) break;
+            R u, vv, t, rho, c5;
+            if (e[k].x < n) {
+                R p;
+                rho = q[k][0];
+                uvtp_from_q<R>(rho, q[k][1], q[k][2], q[k][3], u, vv, t, p, KR);
+                c5 = q[k][4];
+            } else {
+                // a ghost from its parent's state (loaded above as the source cell)
+                const int a = e[k].x & kIdMask, ty = (unsigned)e[k].x >> kTypeShift;
+                Prim<R> pp, s;
+                prim_from_q<R>(pp, q[k][0], q[k][1], q[k][2], q[k][3], q[k][4], KR);
+                R uo = R(0), vo = R(0), to = R(0), po = R(0), ycg = R(0);
+                if (ty == 3)
+                    uvtp_from_q<R>((R)Q[a], (R)Q[nt + a], (R)Q[2 * nt + a], (R)Q[3 * nt + a], uo,
+                                   vo, to, po, KR);
+                else
+                    ycg = (R)F.ycc[a];
+                ghost_state<R>(s, ty, pp, uo, vo, to, po, ycg, KR);
+                u = s.u; vv = s.v; t = s.t; rho = s.c[0]; c5 = s.c[4];
+            }
+            const R rr = fdiv(c5, rho);
+            if (k >= su) au += u;
+            if (k >= sv) av += vv;
+            at += t;
+            arho += rho;
+            if (k >= sr) ar += rr;
         }
-        R rr = fdiv(c5, rho);
-        if (j >= ov.x) au += u;
-        if (j >= ov.y) av += vv;
-        at += t;
-        arho += rho;
-        if (j >= ov.z) ar += rr;
+    } else {
+        ar = F.novr[v].w ? KR.rtol_rtred : R(0);
+        node_gather_list<R>(F, KR, v, au, av, at, arho, ar);
     }
-    R cnt = (R)(j1 - j0);
     const int m = F.m;
-    const Recip<R> rcnt(cnt);
+    const R nc = (R)(cnt <= kNodeWidth ? cnt : F.nptr[v + 1] - F.nptr[v]);
+    const Recip<R> rcnt(nc);
     stv<SD>(F.node, v, rcnt.div(au));
     stv<SD>(F.node, (size_t)m + v, rcnt.div(av));
     stv<SD>(F.node, (size_t)2 * m + v, rcnt.div(at));
@@ -150,8 +212,9 @@ __global__ void __launch_bounds__(kBlock) k_dv(FDev F, Consts<double> K, Consts<
     const double *Q = F.qn;
     double dti = K.local ? F.dt[i] : __longlong_as_double((long long)*F.dtslot);
     if (!K.local) F.dt[i] = dti;
-    int4 nb4 = F.nc4[i], nd4 = F.nod4[i];
-    const int nb[4] = {nb4.x, nb4.y, nb4.z, nb4.w};
+    int4 nbt = F.nct[i], nd4 = F.nod4[i];
+    // raw[f]: neighbour id, with the BC type in the top bits for a ghost
+    const int raw[4] = {nbt.x, nbt.y, nbt.z, nbt.w};
     const int nd[4] = {nd4.x, nd4.y, nd4.z, nd4.w};
     // ---- dissipation (solver_state.f90:183-223); ghost neighbour = parent copy
     double ci[5];
@@ -163,7 +226,7 @@ __global__ void __launch_bounds__(kBlock) k_dv(FDev F, Consts<double> K, Consts<
     for (int q = 0; q < 5; ++q) d2qi[q] = (R)ld<SD>(F.d2q, (size_t)q * nt + i);
     R d2[5], d4[5];
     for (int k = 0; k < 4; ++k) {
-        int j = nb[k] < n ? nb[k] : i;
+        int j = raw[k] < n ? raw[k] : i;
         R dtj = K.local ? (R)F.dt[j] : (R)dti;
         R volj = (R)F.vol[j];
         R a2, a4, b2, b4;
@@ -193,9 +256,9 @@ __global__ void __launch_bounds__(kBlock) k_dv(FDev F, Consts<double> K, Consts<
     bool have_pi = false;
     R vu = R(0), vv = R(0), vt = R(0), vr = R(0);
     for (int f = 0; f < 4; ++f) {
-        int bnode = nd[f], dnode = nd[(f + 1) & 3], c = nb[f];
+        int bnode = nd[f], dnode = nd[(f + 1) & 3], c = raw[f] & kIdMask;
         R uc, vc, tc, rhoc, c5c;
-        if (c < n) {
+        if (raw[f] < n) {
             R pc;
             rhoc = (R)Q[c];
             uvtp_from_q<R>(rhoc, (R)Q[nt + c], (R)Q[2 * nt + c], (R)Q[3 * nt + c], uc, vc, tc, pc, KR);
@@ -205,13 +268,12 @@ __global__ void __launch_bounds__(kBlock) k_dv(FDev F, Consts<double> K, Consts<
                 prim_from_q<R>(pi, rhoa, (R)ci[1], (R)ci[2], (R)ci[3], (R)ci[4], KR);
                 have_pi = true;
             }
-            int k = c - n, ty = F.gtype[k];
+            // type 3: the neighbour across the opposite face, a cell
+            const int ty = (unsigned)raw[f] >> kTypeShift, o = raw[(f + 2) & 3];
             R uo = R(0), vo = R(0), to = R(0), po = R(0);
-            if (ty == 3) {
-                int o = F.gopp[k];
+            if (ty == 3)
                 uvtp_from_q<R>((R)Q[o], (R)Q[nt + o], (R)Q[2 * nt + o], (R)Q[3 * nt + o], uo, vo,
                                to, po, KR);
-            }
             Prim<R> s;
             ghost_state<R>(s, ty, pi, uo, vo, to, po, (R)F.ycc[c], KR);
             uc = s.u; vc = s.v; tc = s.t; rhoc = s.c[0]; c5c = s.c[4];
@@ -257,7 +319,10 @@ __global__ void __launch_bounds__(kBlock) k_rk(FDev F, Consts<double> K, Consts<
     double cnew[5] = {0, 0, 0, 0, 0}, cold[5] = {0, 0, 0, 0, 0};
     const bool active = i < n;
     if (active) {
-        const int4 nb4 = F.nc4[i];
+        // neighbour ids (ghost ids carry their BC type in the top bits)
+        const int4 nbt = F.nct[i];
+        const int4 nb4 = make_int4(nbt.x & kIdMask, nbt.y & kIdMask, nbt.z & kIdMask,
+                                   nbt.w & kIdMask);
         const double *Q = F.qn;
         double dti;
         if (STAGE == 1) {
@@ -308,6 +373,19 @@ __global__ void __launch_bounds__(kBlock) k_rk(FDev F, Consts<double> K, Consts<
             for (int q = 0; q < 4; ++q) vs[q] = ld<SD>(F.vis, (size_t)q * n + i);
         }
 
+        // FP32: every face's neighbour state is loaded up front (a ghost face
+        // loads this cell again), so no load waits on a branch.  FP64 loads
+        // them face by face: holding all twenty would cost the second
+        // wavefront per SIMD (256 VGPRs).
+        constexpr bool kPreload = sizeof(R) == 4;
+        R qf[4][5];
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+            const int jn = sel4(f, nb4.x, nb4.y, nb4.z, nb4.w);
+            const int j = jn < n ? jn : i;
+#pragma unroll
+            for (int q = 0; q < 5; ++q) qf[f][q] = kPreload ? (R)ld<SI>(qin, q * sin + j) : R(0);
+        }
         Prim<R> pi;
         prim_from_q<R>(pi, (R)ld<SI>(qin, i), (R)ld<SI>(qin, sin + i), (R)ld<SI>(qin, 2 * sin + i),
                        (R)ld<SI>(qin, 3 * sin + i), (R)ld<SI>(qin, 4 * sin + i), KR);
@@ -318,20 +396,31 @@ __global__ void __launch_bounds__(kBlock) k_rk(FDev F, Consts<double> K, Consts<
 #pragma unroll
         for (int f = 0; f < 4; ++f) {
             const int jn = sel4(f, nb4.x, nb4.y, nb4.z, nb4.w);
+            const int ty = (unsigned)sel4(f, nbt.x, nbt.y, nbt.z, nbt.w) >> kTypeShift;
             Prim<R> pj;
-            if (jn < n) {
-                prim_from_q<R>(pj, (R)ld<SI>(qin, jn), (R)ld<SI>(qin, sin + jn),
-                               (R)ld<SI>(qin, 2 * sin + jn), (R)ld<SI>(qin, 3 * sin + jn),
-                               (R)ld<SI>(qin, 4 * sin + jn), KR);
+            if (ty == 0) {
+                if (kPreload)
+                    prim_from_q<R>(pj, qf[f][0], qf[f][1], qf[f][2], qf[f][3], qf[f][4], KR);
+                else
+                    prim_from_q<R>(pj, (R)ld<SI>(qin, jn), (R)ld<SI>(qin, sin + jn),
+                                   (R)ld<SI>(qin, 2 * sin + jn), (R)ld<SI>(qin, 3 * sin + jn),
+                                   (R)ld<SI>(qin, 4 * sin + jn), KR);
             } else {
-                // the ghost across this face: only this cell reads it (boundary.f90)
-                const int k = jn - n, ty = F.gtype[k];
+                // the ghost across this face: only this cell reads it (boundary.f90);
+                // a type-3 ghost extrapolates from the neighbour across the
+                // opposite face, which is a cell (checked when the mesh is set up)
+                const int fo = (f + 2) & 3;
                 R uo = R(0), vo = R(0), to = R(0), po = R(0);
                 if (ty == 3) {
-                    const int o = sel4((f + 2) & 3, nb4.x, nb4.y, nb4.z, nb4.w);
-                    uvtp_from_q<R>((R)ld<SI>(qin, o), (R)ld<SI>(qin, sin + o),
-                                   (R)ld<SI>(qin, 2 * sin + o), (R)ld<SI>(qin, 3 * sin + o), uo,
-                                   vo, to, po, KR);
+                    if (kPreload) {
+                        uvtp_from_q<R>(qf[fo][0], qf[fo][1], qf[fo][2], qf[fo][3], uo, vo, to, po,
+                                       KR);
+                    } else {
+                        const int o = sel4(fo, nb4.x, nb4.y, nb4.z, nb4.w);
+                        uvtp_from_q<R>((R)ld<SI>(qin, o), (R)ld<SI>(qin, sin + o),
+                                       (R)ld<SI>(qin, 2 * sin + o), (R)ld<SI>(qin, 3 * sin + o),
+                                       uo, vo, to, po, KR);
+                    }
                 }
                 ghost_state<R>(pj, ty, pi, uo, vo, to, po, (R)F.ycc[jn], KR);
             }
@@ -449,7 +538,8 @@ hipError_t enqueue_iteration(tdt_ctx *c, FusedBufs &B, const Consts<R> &KR) {
     const FDev &F = B.F;
     if (B.variant == kVariantQuad) {
         const unsigned nq = B.nbq_cells;
-        launch(c, B, 0, kq_pre<SD, R>, nq + B.nbq_nodes, F, c->K, KR, B.nbq_cells, B.nbq_cells);
+        launch(c, B, 0, kq_pre<SD, R>, nq + B.nbq_nodes + 1, F, c->K, KR, B.nbq_cells,
+               B.nbq_cells);
         if (c->profiling && B.have_events) {
             (void)hipEventRecord(B.ev[2], c->stream);
             (void)hipEventRecord(B.ev[3], c->stream);
@@ -461,7 +551,7 @@ hipError_t enqueue_iteration(tdt_ctx *c, FusedBufs &B, const Consts<R> &KR) {
         return hipGetLastError();
     }
     const unsigned nbc = B.nb_cells;
-    launch(c, B, 0, k_pre<SD, R>, nbc + B.nb_nodes, F, c->K, KR, B.nb_cells);
+    launch(c, B, 0, k_pre<SD, R>, nbc + B.nb_nodes + 1, F, c->K, KR, B.nb_cells);
     if (B.variant == kVariantSplit) {
         launch(c, B, 1, k_dv<SD, R>, nbc, F, c->K, KR);
         launch(c, B, 2, k_rk<R, double, SS, SD, 1, false>, nbc, F, c->K, KR, 0.25);

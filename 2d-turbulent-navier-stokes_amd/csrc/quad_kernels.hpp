@@ -16,13 +16,17 @@ namespace {
 constexpr int kCellsPerBlock = kBlock / 4;
 
 // ---------------------------------------------------------------------------------
-// kq_pre: blocks [0, nbq_cells) do cells, the rest do nodes (64 per block).
+// kq_pre: blocks [0, nbq_cells) do cells, the rest do nodes (64 per block); the
+// last block folds the previous iteration's monitor partials.
 template <class SD, class R>
 __global__ void __launch_bounds__(kBlock) kq_pre(FDev F, Consts<double> K, Consts<R> KR,
                                                  int nbq_cells, int nb_partials) {
     const int lane = threadIdx.x & 3;
     const int b = blockIdx.x;
-    if (b == 0) fold_monitor(F, nb_partials);
+    if (b == (int)gridDim.x - 1) {
+        fold_monitor(F, nb_partials);
+        return;
+    }
     const int nt = F.nt, n = F.n;
     const double *Q = F.qn;
     if (b < nbq_cells) {

@@ -1,6 +1,6 @@
-// Deterministic reductions: 64-lane wavefront shuffles, then a fixed-order
-// combination of the block's waves.  Sums are reproducible run to run because
-// the grid shape fixes the order; min / max are order-independent anyway.
+// Deterministic reductions: 64-lane wavefront DPP butterflies, then a
+// fixed-order combination of the block's waves.  Sums are reproducible run to
+// run because the grid shape fixes the order; min / max are order-independent.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -10,28 +10,43 @@ namespace tdt {
 constexpr int kWave = 64;
 constexpr int kPartStride = 11;  // per block: 5 sums of squares, 5 maxima, 1 flag
 
+// Value of x in lane (i - K) mod 16 of this lane's row of 16 (DPP row_ror:K,
+// a VALU move: no LDS round trip).
+template <int K>
+__device__ __forceinline__ double row_ror(double x) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), 0x120 + K, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), 0x120 + K, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double lane_value(double x, int lane) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(x), lane),
+                            __builtin_amdgcn_readlane(__double2loint(x), lane));
+}
+
+// Wavefront reductions (all 64 lanes active): a rotate-and-combine butterfly
+// inside each row of 16 lanes, then the four row results (lanes 0, 16, 32, 48)
+// in row order.  The result is uniform; the order is fixed, so sums repeat
+// bit for bit.
+template <class Op>
+__device__ __forceinline__ double wave_reduce(double x, Op op) {
+    x = op(x, row_ror<8>(x));
+    x = op(x, row_ror<4>(x));
+    x = op(x, row_ror<2>(x));
+    x = op(x, row_ror<1>(x));
+    return op(op(op(lane_value(x, 0), lane_value(x, 16)), lane_value(x, 32)), lane_value(x, 48));
+}
+
 __device__ __forceinline__ double wave_sum(double x) {
-#pragma unroll
-    for (int off = kWave / 2; off > 0; off >>= 1) x += __shfl_down(x, off, kWave);
-    return x;
+    return wave_reduce(x, [](double a, double b) { return a + b; });
 }
 
 __device__ __forceinline__ double wave_max(double x) {
-#pragma unroll
-    for (int off = kWave / 2; off > 0; off >>= 1) {
-        double y = __shfl_down(x, off, kWave);
-        x = y > x ? y : x;
-    }
-    return x;
+    return wave_reduce(x, [](double a, double b) { return b > a ? b : a; });
 }
 
 __device__ __forceinline__ double wave_min(double x) {
-#pragma unroll
-    for (int off = kWave / 2; off > 0; off >>= 1) {
-        double y = __shfl_down(x, off, kWave);
-        x = y < x ? y : x;
-    }
-    return x;
+    return wave_reduce(x, [](double a, double b) { return b < a ? b : a; });
 }
 
 // Block minimum of non-negative values folded into *slot with an atomic min on

@@ -20,7 +20,8 @@ import numpy as np
 from .caseio import Flow, Mesh
 
 PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(PKG_DIR, "lib", "libtdt.so")
+# TDT_LIBRARY selects another build of the library (e.g. to compare two builds)
+LIB_PATH = os.environ.get("TDT_LIBRARY") or os.path.join(PKG_DIR, "lib", "libtdt.so")
 HEADER_PATH = os.path.join(os.path.dirname(PKG_DIR), "include", "tdt.h")
 
 TDT_OK, TDT_ERR_ARG, TDT_ERR_MESH, TDT_ERR_DEVICE, TDT_ERR_NONFINITE, TDT_ERR_UNSUPPORTED = (
