@@ -430,6 +430,9 @@ int tdt_set_array(tdt_ctx *c, const char *name, const double *in, int32_t len) {
             for (int k = 0; k < pm.g; ++k) h[pm.n + k] = in[pm.ghost_orig_id[k] - 1];
     }
     HIPCALL(c, hipMemcpyAsync(a.p, h.data(), cnt * sizeof(double), hipMemcpyHostToDevice, c->stream), "set");
+    const std::string nm(name);
+    if (nm == "XCC" || nm == "YCC" || nm == "VOL" || nm == "DL" || nm.rfind("SC", 0) == 0)
+        ++c->geom_version;   // the fused path refreshes its copies
     return sync(c);
 }
 

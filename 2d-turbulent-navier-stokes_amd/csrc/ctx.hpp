@@ -67,6 +67,7 @@ struct tdt_ctx {
     tdt::FusedBufs *fused = nullptr;
     int profiling = 0;
     double ktimes[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int geom_version = 0;     // bumped when tdt_set_array changes geometry
 };
 
 namespace tdt {

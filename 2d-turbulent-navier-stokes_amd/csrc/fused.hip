@@ -41,6 +41,11 @@ __global__ void k_reset_counters(int *ctr, unsigned long long *dtslot) {
     *dtslot = (unsigned long long)__double_as_longlong(100000.0);
 }
 
+__global__ void __launch_bounds__(kBlock) k_narrow(const double *src, float *dst, int count) {
+    int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < count) dst[i] = (float)src[i];
+}
+
 __global__ void __launch_bounds__(kBlock) k_cold_from_qn(FDev F, double *cold) {
     int i = blockIdx.x * kBlock + threadIdx.x;
     if (i < F.n)
@@ -72,6 +77,7 @@ int fused_setup(tdt_ctx *c) {
     F.gparent = D.gparent; F.gtype = D.gtype; F.gopp = D.gopp;
     F.nptr = D.nptr; F.nlist = D.nlist; F.novr = D.novr;
     F.xcc = D.xcc; F.ycc = D.ycc; F.vol = D.vol; F.dl = D.dl; F.sc = D.sc;
+    F.gxcc = D.xcc; F.gycc = D.ycc; F.gvol = D.vol; F.gsc = D.sc; F.gxn = D.xn; F.gyn = D.yn;
     F.qn = D.c;
     F.dt = D.dt;
     bool ok = true;
@@ -92,6 +98,13 @@ int fused_setup(tdt_ctx *c) {
         F.st[0] = dalloc<float>(*B, (size_t)5 * D.n);
         F.st[1] = dalloc<float>(*B, (size_t)5 * D.n);
         ok = F.anu && F.anurt && F.d2q && F.dd && F.vis && F.node && F.st[0] && F.st[1];
+        F.gxcc = dalloc<float>(*B, D.nt);
+        F.gycc = dalloc<float>(*B, D.nt);
+        F.gvol = dalloc<float>(*B, D.nt);
+        F.gsc = dalloc<float>(*B, (size_t)8 * D.n);
+        F.gxn = dalloc<float>(*B, D.m);
+        F.gyn = dalloc<float>(*B, D.m);
+        ok = ok && F.gxcc && F.gycc && F.gvol && F.gsc && F.gxn && F.gyn;
     }
     B->nb_cells = (int)nblocks(D.n);
     B->nb_nodes = (int)nblocks(D.m);
@@ -246,6 +259,22 @@ int fused_iterate(tdt_ctx *c, int niter, double *l2, double *linf, int *first_no
     if (rc != TDT_OK) return rc;
     FusedBufs &B = *c->fused;
     FDev &F = B.F;
+    if (c->precision != TDT_FP64 && B.geom_version != c->geom_version) {
+        // FP32 copies of the geometry the flux arithmetic reads
+        const Dev &D = c->D;
+        const struct { const double *s; const void *d; int cnt; } cp[] = {
+            {D.xcc, F.gxcc, D.nt}, {D.ycc, F.gycc, D.nt}, {D.vol, F.gvol, D.nt},
+            {D.sc, F.gsc, 8 * D.n}, {D.xn, F.gxn, D.m}, {D.yn, F.gyn, D.m}};
+        for (const auto &x : cp)
+            hipLaunchKernelGGL(k_narrow, dim3(nblocks(x.cnt)), dim3(kBlock), 0, c->stream, x.s,
+                               (float *)x.d, x.cnt);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) {
+            c->err = std::string("geometry copies: ") + hipGetErrorString(e);
+            return TDT_ERR_DEVICE;
+        }
+        B.geom_version = c->geom_version;
+    }
     if ((size_t)niter > B.hist_cap) {
         size_t cap = std::max((size_t)niter, (size_t)1024);
         double *h = dalloc<double>(B, cap * 10);

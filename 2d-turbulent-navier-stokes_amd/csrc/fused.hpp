@@ -38,6 +38,9 @@ struct FDev {
     const int *nptr, *nlist;
     const int4 *novr;
     const double *xcc, *ycc, *vol, *dl, *sc;
+    // geometry as the flux arithmetic reads it, in the storage type SD: the FP64
+    // arrays themselves, or FP32 copies rounded from them (FP32-mixed)
+    const void *gxcc, *gycc, *gvol, *gsc, *gxn, *gyn;
     double *qn;                     // [5][nt] Q^n (the C array), FP64
     void *st[2];                    // [5][n] stage states Q(1), Q(2)/Q(3)
     double *dt;                     // [n]
@@ -67,6 +70,7 @@ struct FusedBufs {
     bool tuned = false;              // variant picked by timing (autotune)
     double *qbak = nullptr;          // Q^n saved while candidates are timed
     int nbq_cells = 0, nbq_nodes = 0;
+    int geom_version = -1;           // tdt_ctx::geom_version the FP32 copies match
 };
 
 // Kernel sets of one iteration; all give bitwise-identical results.

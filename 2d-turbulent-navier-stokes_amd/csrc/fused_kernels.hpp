@@ -176,7 +176,7 @@ __global__ void __launch_bounds__(kBlock) k_pre(FDev F, Consts<double> K, Consts
                     uvtp_from_q<R>((R)Q[a], (R)Q[nt + a], (R)Q[2 * nt + a], (R)Q[3 * nt + a], uo,
                                    vo, to, po, KR);
                 else
-                    ycg = (R)F.ycc[a];
+                    ycg = (R)ld<SD>(F.gycc, a);
                 ghost_state<R>(s, ty, pp, uo, vo, to, po, ycg, KR);
                 u = s.u; vv = s.v; t = s.t; rho = s.c[0]; c5 = s.c[4];
             }
@@ -219,7 +219,7 @@ __global__ void __launch_bounds__(kBlock) k_dv(FDev F, Consts<double> K, Consts<
     // ---- dissipation (solver_state.f90:183-223); ghost neighbour = parent copy
     double ci[5];
     for (int q = 0; q < 5; ++q) ci[q] = Q[q * nt + i];
-    R voli = (R)F.vol[i];
+    R voli = (R)ld<SD>(F.gvol, i);
     R volbydt = fdiv(voli, (R)dti);
     R anui = (R)ld<SD>(F.anu, i), anurti = (R)ld<SD>(F.anurt, i);
     R d2qi[5];
@@ -228,7 +228,7 @@ __global__ void __launch_bounds__(kBlock) k_dv(FDev F, Consts<double> K, Consts<
     for (int k = 0; k < 4; ++k) {
         int j = raw[k] < n ? raw[k] : i;
         R dtj = K.local ? (R)F.dt[j] : (R)dti;
-        R volj = (R)F.vol[j];
+        R volj = (R)ld<SD>(F.gvol, j);
         R a2, a4, b2, b4;
         jst_coeffs<R>(anui, (R)ld<SD>(F.anu, j), volbydt, volj, dtj, a2, a4);
         jst_coeffs<R>(anurti, (R)ld<SD>(F.anurt, j), volbydt, volj, dtj, b2, b4);
@@ -251,7 +251,7 @@ __global__ void __launch_bounds__(kBlock) k_dv(FDev F, Consts<double> K, Consts<
     R rhoa = (R)ci[0];
     uvtp_from_q<R>(rhoa, (R)ci[1], (R)ci[2], (R)ci[3], ua, va, ta, pa, KR);
     R ra = fdiv((R)ci[4], rhoa);
-    R xa = (R)F.xcc[i], ya = (R)F.ycc[i];
+    R xa = (R)ld<SD>(F.gxcc, i), ya = (R)ld<SD>(F.gycc, i);
     Prim<R> pi;
     bool have_pi = false;
     R vu = R(0), vv = R(0), vt = R(0), vr = R(0);
@@ -275,13 +275,14 @@ __global__ void __launch_bounds__(kBlock) k_dv(FDev F, Consts<double> K, Consts<
                 uvtp_from_q<R>((R)Q[o], (R)Q[nt + o], (R)Q[2 * nt + o], (R)Q[3 * nt + o], uo, vo,
                                to, po, KR);
             Prim<R> s;
-            ghost_state<R>(s, ty, pi, uo, vo, to, po, (R)F.ycc[c], KR);
+            ghost_state<R>(s, ty, pi, uo, vo, to, po, (R)ld<SD>(F.gycc, c), KR);
             uc = s.u; vc = s.v; tc = s.t; rhoc = s.c[0]; c5c = s.c[4];
         }
         R rc = fdiv(c5c, rhoc);
         ViscFace<R> o = viscous_face<R>(
-            (R)F.sc[(2 * f) * n + i], (R)F.sc[(2 * f + 1) * n + i], xa, ya, (R)F.xn[bnode],
-            (R)F.yn[bnode], (R)F.xcc[c], (R)F.ycc[c], (R)F.xn[dnode], (R)F.yn[dnode], ua, va, ta,
+            (R)ld<SD>(F.gsc, (2 * f) * n + i), (R)ld<SD>(F.gsc, (2 * f + 1) * n + i), xa, ya,
+            (R)ld<SD>(F.gxn, bnode), (R)ld<SD>(F.gyn, bnode), (R)ld<SD>(F.gxcc, c),
+            (R)ld<SD>(F.gycc, c), (R)ld<SD>(F.gxn, dnode), (R)ld<SD>(F.gyn, dnode), ua, va, ta,
             rhoa, ra, (R)ld<SD>(F.node, bnode), (R)ld<SD>(F.node, (size_t)m + bnode),
             (R)ld<SD>(F.node, (size_t)2 * m + bnode), (R)ld<SD>(F.node, (size_t)3 * m + bnode),
             (R)ld<SD>(F.node, (size_t)4 * m + bnode), uc, vc, tc, rhoc, rc,
@@ -342,14 +343,14 @@ __global__ void __launch_bounds__(kBlock) k_rk(FDev F, Consts<double> K, Consts<
                 d2qi[q] = (R)ld<SD>(F.d2q, (size_t)q * nt + i);
             }
             const R anui = (R)ld<SD>(F.anu, i), anurti = (R)ld<SD>(F.anurt, i);
-            const R volbydt = fdiv((R)F.vol[i], (R)dti);
+            const R volbydt = fdiv((R)ld<SD>(F.gvol, i), (R)dti);
             R d2[5], d4[5];
 #pragma unroll
             for (int f = 0; f < 4; ++f) {
                 const int jn = sel4(f, nb4.x, nb4.y, nb4.z, nb4.w);
                 const int j = jn < n ? jn : i;
                 const R dtj = K.local ? (R)F.dt[j] : (R)dti;
-                const R volj = (R)F.vol[j];
+                const R volj = (R)ld<SD>(F.gvol, j);
                 R a2, a4, b2, b4;
                 jst_coeffs<R>(anui, (R)ld<SD>(F.anu, j), volbydt, volj, dtj, a2, a4);
                 jst_coeffs<R>(anurti, (R)ld<SD>(F.anurt, j), volbydt, volj, dtj, b2, b4);
@@ -422,17 +423,18 @@ __global__ void __launch_bounds__(kBlock) k_rk(FDev F, Consts<double> K, Consts<
                                        uo, vo, to, po, KR);
                     }
                 }
-                ghost_state<R>(pj, ty, pi, uo, vo, to, po, (R)F.ycc[jn], KR);
+                ghost_state<R>(pj, ty, pi, uo, vo, to, po, (R)ld<SD>(F.gycc, jn), KR);
             }
             // central flux terms of this face (residual.f90:34-54)
-            const R sx = (R)F.sc[(2 * f) * n + i], sy = (R)F.sc[(2 * f + 1) * n + i];
+            const R sx = (R)ld<SD>(F.gsc, (2 * f) * n + i);
+            const R sy = (R)ld<SD>(F.gsc, (2 * f + 1) * n + i);
 #pragma unroll
             for (int q = 0; q < 5; ++q) {
                 const R tf = (pi.f[q] + pj.f[q]) * sx;
                 const R tg = (pi.g[q] + pj.g[q]) * sy;
                 acc[q] = f == 0 ? tf + tg : acc[q] + tf + tg;
             }
-            const R xj = (R)F.xcc[jn], yj = (R)F.ycc[jn];
+            const R xj = (R)ld<SD>(F.gxcc, jn), yj = (R)ld<SD>(F.gycc, jn);
             const SrcNbr<R> nbv = src_nbr<R>(xj, yj, pj.u, pj.v, pj.c[0], pj.c[4]);
             if (f == 0) sa = nbv;
             if (f == 1) sb = nbv;
@@ -444,8 +446,9 @@ __global__ void __launch_bounds__(kBlock) k_rk(FDev F, Consts<double> K, Consts<
                 const int dn = sel4(f, nd4.y, nd4.z, nd4.w, nd4.x);
                 const int m = F.m;
                 const ViscFace<R> o = viscous_face<R>(
-                    sx, sy, (R)F.xcc[i], (R)F.ycc[i], (R)F.xn[bn], (R)F.yn[bn], xj, yj,
-                    (R)F.xn[dn], (R)F.yn[dn], pi.u, pi.v, pi.t, pi.c[0], fdiv(pi.c[4], pi.c[0]),
+                    sx, sy, (R)ld<SD>(F.gxcc, i), (R)ld<SD>(F.gycc, i), (R)ld<SD>(F.gxn, bn),
+                    (R)ld<SD>(F.gyn, bn), xj, yj, (R)ld<SD>(F.gxn, dn), (R)ld<SD>(F.gyn, dn),
+                    pi.u, pi.v, pi.t, pi.c[0], fdiv(pi.c[4], pi.c[0]),
                     (R)ld<SD>(F.node, bn), (R)ld<SD>(F.node, (size_t)m + bn),
                     (R)ld<SD>(F.node, (size_t)2 * m + bn), (R)ld<SD>(F.node, (size_t)3 * m + bn),
                     (R)ld<SD>(F.node, (size_t)4 * m + bn), pj.u, pj.v, pj.t, pj.c[0],
@@ -465,7 +468,8 @@ __global__ void __launch_bounds__(kBlock) k_rk(FDev F, Consts<double> K, Consts<
             for (int q = 0; q < 4; ++q) stv<SD>(F.vis, (size_t)q * n + i, vs[q]);
         }
         R s3;
-        const R s5 = source_terms_d<R>(sd, pi.c[0], pi.c[4], pi.t, pi.p, pi.v, (R)F.ycc[i], s3, KR);
+        const R s5 = source_terms_d<R>(sd, pi.c[0], pi.c[4], pi.t, pi.p, pi.v,
+                                       (R)ld<SD>(F.gycc, i), s3, KR);
         // residual.f90:34-60 and the update residual.f90:76-81 (FP64)
         const double vol = F.vol[i];
         double r[5];

@@ -228,3 +228,23 @@ def test_autotune_is_transparent(golden, monkeypatch):
         assert np.max(rel) <= 1e-13
     else:
         assert np.array_equal(out[None][1], out["merged"][1])
+
+
+def test_fp32_geometry_copies_follow_set_array(golden):
+    """The FP32-mixed kernels read FP32 copies of the geometry; changing the
+    geometry through tdt_set_array after a first iterate must reach them."""
+    import tdt
+    g = golden("case1")
+    out = {}
+    for prec in ("fp64", "fp32-mixed"):
+        with tdt.Solver(g.mesh, g.flow, prec) as s:
+            s.cold_start()
+            s.iterate(3)
+            vol = s.get_array("VOL")
+            s.set_array("VOL", vol * 1.25)
+            sc = s.get_array("SC1X")
+            s.set_array("SC1X", sc * 1.01)
+            s.iterate(30)
+            out[prec] = s.get_state()
+    errs = field_errors(out["fp32-mixed"], out["fp64"], g.flow)
+    assert errs["p"] < 1e-5 and errs["rho"] < 1e-5, errs
