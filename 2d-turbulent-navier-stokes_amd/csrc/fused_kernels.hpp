@@ -17,6 +17,12 @@ __device__ __forceinline__ int logical_block(int remap) {
     return xcd < rem ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
 }
 
+// Logical block within a range of nb8 blocks (nb8 and the range's first block
+// multiples of 8): each XCD gets a contiguous eighth of the range.
+__device__ __forceinline__ int xcd_chunk(int local, int nb8) {
+    return (local & 7) * (nb8 >> 3) + (local >> 3);
+}
+
 template <class T>
 __device__ __forceinline__ double ld(const void *p, size_t i) {
     return (double)static_cast<const T *>(p)[i];
@@ -84,18 +90,22 @@ __device__ __forceinline__ void node_gather_list(const FDev &F, const Consts<R> 
 }
 
 // ---------------------------------------------------------------------------------
-// k_pre: cells [0, nb_cells) blocks, nodes after, and a last block of its own
-// that folds the previous iteration's monitor partials (off every cell's path).
+// k_pre: cell blocks [0, ncb8), node blocks [ncb8, ncb8 + nnb8) (both padded
+// to multiples of 8 and spread so that each XCD works on contiguous cells and
+// the nodes around them), and a last block of its own that folds the previous
+// iteration's monitor partials (off every cell's path).
 template <class SD, class R>
 __global__ void __launch_bounds__(kBlock) k_pre(FDev F, Consts<double> K, Consts<R> KR,
-                                                int nb_cells) {
+                                                int nb_cells, int ncb8, int nnb8) {
     int b = blockIdx.x;
     if (b == (int)gridDim.x - 1) {
         fold_monitor(F, nb_cells);
         return;
     }
-    if (b < nb_cells) {
-        int i = b * kBlock + threadIdx.x;
+    if (b < ncb8) {
+        const int cb = F.remap ? xcd_chunk(b, ncb8) : b;
+        if (cb >= nb_cells) return;
+        int i = cb * kBlock + threadIdx.x;
         double dtl = 100000.0;
         if (i < F.n) {
             const int nt = F.nt;
@@ -137,7 +147,8 @@ __global__ void __launch_bounds__(kBlock) k_pre(FDev F, Consts<double> K, Consts
         if (!K.local) block_atomic_min_pos(dtl, reinterpret_cast<double *>(F.dtslot));
         return;
     }
-    int v = (b - nb_cells) * kBlock + threadIdx.x;
+    const int nbl = F.remap ? xcd_chunk(b - ncb8, nnb8) : b - ncb8;
+    int v = nbl * kBlock + threadIdx.x;
     if (v >= F.m) return;
     const int hr = F.nrel[v];
     const int cnt = hr & 15;
@@ -555,7 +566,8 @@ hipError_t enqueue_iteration(tdt_ctx *c, FusedBufs &B, const Consts<R> &KR) {
         return hipGetLastError();
     }
     const unsigned nbc = B.nb_cells;
-    launch(c, B, 0, k_pre<SD, R>, nbc + B.nb_nodes + 1, F, c->K, KR, B.nb_cells);
+    const int ncb8 = (B.nb_cells + 7) & ~7, nnb8 = (B.nb_nodes + 7) & ~7;
+    launch(c, B, 0, k_pre<SD, R>, ncb8 + nnb8 + 1, F, c->K, KR, B.nb_cells, ncb8, nnb8);
     if (B.variant == kVariantSplit) {
         launch(c, B, 1, k_dv<SD, R>, nbc, F, c->K, KR);
         launch(c, B, 2, k_rk<R, double, SS, SD, 1, false>, nbc, F, c->K, KR, 0.25);

@@ -42,23 +42,29 @@ def alg_bytes_per_cell(precision: str, m_over_n: float, local: int, kset: str = 
 
     Every array a kernel reads or writes is counted once at its storage width;
     neighbour and node re-reads are cache hits; ghost work (O(perimeter)) is
-    neglected.  s = width of stage / dissipation / viscous / node storage.
+    neglected.  s: width of stage states, D, VIS, sensors and node averages; g:
+    width of the geometry the flux arithmetic reads (FP32 copies in the mixed
+    path, whose Q update still reads the FP64 volume).  Q^n and DT are FP64.
     The merged and quad kernel sets have no k_dv launch: stage 1 computes the
     dissipation and viscous fluxes itself, so it moves k_dv's bytes plus its
     stage output.
     """
-    s = 8 if precision == "fp64" else 4
-    q, nc, nod, sc, vol, xy, dt, dl = 40, 16, 16, 64, 8, 16, 8, 8
-    pre_cells = q + dl + nc + (dt if local else 0) + 7 * s          # Q, DL, NC -> DT, ANU, ANURT, D2Q
-    pre_nodes = m_over_n * (4 + 16 + 16 + 5 * s)                    # CSR ptr, overrides, ~4 ids -> 5 avgs
-    dv = (q + dt + vol + 2 * s + 5 * s + nc + nod + sc + xy + m_over_n * (16 + 5 * s)
-          + 5 * s + 4 * s)                                          # -> D, VIS
-    common = nc + sc + vol + xy + dt + 5 * s + 4 * s                # geometry, DT, D, VIS
+    s = g = 8 if precision == "fp64" else 4
+    q, nc, nod, dt, dl = 40, 16, 16, 8, 8
+    sc, xy = 8 * g, 2 * g
+    vol_upd = 8 if precision != "fp64" else 0           # FP64 volume besides the FP32 copy
+    pre_cells = q + dl + nc + (dt if local else 0) + 7 * s   # -> DT, ANU, ANURT, D2Q
+    pre_nodes = m_over_n * (4 + 8 * 4 + 5 * s)               # records -> 5 averages
+    dv = (q + dt + g + 2 * s + 5 * s + nc + nod + sc + xy + m_over_n * (2 * g + 5 * s)
+          + 5 * s + 4 * s)                                   # -> D, VIS
+    common = nc + sc + xy + dt + g + vol_upd + 5 * s + 4 * s   # geometry, DT, D, VIS
+    if precision == "fp64":
+        common -= g                                          # one volume array
     st1 = q + common + 5 * s
     st23 = 5 * s + q + common + 5 * s
     st4 = 5 * s + q + common + q
     if kset != "split":
-        dv, st1 = 0.0, dv + 5 * s
+        dv, st1 = 0.0, dv + vol_upd + 5 * s
     return {"k_pre": pre_cells + pre_nodes, "k_dv": dv, "k_stage1": st1, "k_stage2": st23,
             "k_stage3": st23, "k_stage4": st4}
 
@@ -73,6 +79,27 @@ def load_traffic(workload: str, precision: str, kset: str):
     with open(path) as fh:
         d = json.load(fh).get(precision)
     return d if d and d.get("kernel_set") == kset else None
+
+
+def roofline(workload, precision, mesh, flow, kt, kset, elapsed, steps):
+    """HBM roofline of the dominant kernel (largest average duration per launch):
+    algorithmic bytes per launch / measured average launch time."""
+    n = mesh.neles
+    b = alg_bytes_per_cell(precision, mesh.nods / n, flow.local, kset)
+    dom = int(np.argmax(kt))
+    name = KERNELS[dom]
+    achieved = b[name] * n / (kt[dom] * 1e-3) / 1e9
+    traffic = load_traffic(workload, precision, kset)
+    return {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": (traffic or {}).get(name),
+            "alg_bytes_per_cell": round(b[name], 1), "cells_per_launch": n,
+            "avg_launch_ms": round(float(kt[dom]), 5),
+            "kernel_set": kset,
+            "alg_bytes_per_cell_iter": round(sum(b.values()), 1),
+            "iteration_alg_GBs": round(sum(b.values()) * n * steps / elapsed / 1e9, 1),
+            "iteration_frac": round(sum(b.values()) * n * steps / elapsed / 1e9 / HBM_PEAK_GBS,
+                                    4)}
 
 
 def run_gpu(mesh, flow, precision, steps, warmup, ordering="auto"):
@@ -213,20 +240,10 @@ def main():
         print(f"bench: fp32-mixed {v32:.1f} Mcell-iters/s, rel err vs fp64 {err}", file=sys.stderr,
               flush=True)
 
-    # roofline of the dominant kernel (largest average duration per launch)
-    b = alg_bytes_per_cell("fp64", mesh.nods / n, flow.local, ks64)
-    dom = int(np.argmax(kt64))
-    name = KERNELS[dom]
-    t_launch = kt64[dom] * 1e-3
-    achieved = b[name] * n / t_launch / 1e9
-    traffic = load_traffic(args.workload, "fp64", ks64)
-    roof = {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": (traffic or {}).get(name),
-            "alg_bytes_per_cell": round(b[name], 1), "cells_per_launch": n,
-            "avg_launch_ms": round(kt64[dom], 5),
-            "alg_bytes_per_cell_iter": round(sum(b.values()), 1),
-            "iteration_alg_GBs": round(sum(b.values()) * n * args.steps / el64 / 1e9, 1)}
+    roof = roofline(args.workload, "fp64", mesh, flow, kt64, ks64, el64, args.steps)
+    if mixed is not None:
+        mixed["roofline"] = roofline(args.workload, "fp32-mixed", mesh, flow, kt32, ks32, el32,
+                                     args.steps)
 
     cpu = None if args.no_cpu else cpu_baseline(mesh, flow)
     line = {
