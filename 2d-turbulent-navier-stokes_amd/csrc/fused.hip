@@ -107,6 +107,7 @@ int fused_setup(tdt_ctx *c) {
         ok = ok && F.gxcc && F.gycc && F.gvol && F.gsc && F.gxn && F.gyn;
     }
     B->nb_cells = (int)nblocks(D.n);
+    F.ncb8 = (B->nb_cells + 7) & ~7;
     B->nb_nodes = (int)nblocks(D.m);
     B->nbq_cells = (D.n + kBlock / 4 - 1) / (kBlock / 4);
     B->nbq_nodes = (D.m + kBlock / 4 - 1) / (kBlock / 4);

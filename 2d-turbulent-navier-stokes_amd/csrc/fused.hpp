@@ -26,6 +26,7 @@ constexpr int kNodeWidth = 4;
 struct FDev {
     int n, g, m, nt;
     int remap;                      // XCD-aware block order
+    int ncb8;                       // cell blocks rounded up to a multiple of 8
     const int4 *nc4, *nod4;
     const int4 *nct;                // [n] nc4 with ghost ids | type << kTypeShift
     const int2 *ncon;               // [m][kNodeWidth] contributors {a, source cell}: a =

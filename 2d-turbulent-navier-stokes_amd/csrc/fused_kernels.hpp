@@ -8,8 +8,16 @@
 namespace tdt {
 namespace {
 
-// Logical block index so that each XCD (blocks are dispatched round-robin over
-// the 8 XCDs) works on a contiguous range of cells and its L2 sees the halo.
+// Logical block within a range of nb8 blocks (nb8 and the range's first block
+// multiples of 8).  Blocks are dispatched round-robin over the 8 XCDs, so this
+// gives each XCD a contiguous eighth of the range: its L2 then holds most of
+// the neighbours and nodes its cells gather.
+__device__ __forceinline__ int xcd_chunk(int local, int nb8) {
+    return (local & 7) * (nb8 >> 3) + (local >> 3);
+}
+
+// Logical block for a grid of any size (the quad kernels): XCD x gets blocks
+// [x * per + min(x, rem), ...) with per = nb / 8, rem = nb % 8.
 __device__ __forceinline__ int logical_block(int remap) {
     int b = blockIdx.x, nb = gridDim.x;
     if (!remap || nb < 16) return b;
@@ -17,10 +25,10 @@ __device__ __forceinline__ int logical_block(int remap) {
     return xcd < rem ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
 }
 
-// Logical block within a range of nb8 blocks (nb8 and the range's first block
-// multiples of 8): each XCD gets a contiguous eighth of the range.
-__device__ __forceinline__ int xcd_chunk(int local, int nb8) {
-    return (local & 7) * (nb8 >> 3) + (local >> 3);
+// Logical cell block of a cell kernel launched on F.ncb8 blocks (the blocks
+// past the last cell block have no cells).
+__device__ __forceinline__ int cell_block(const FDev &F) {
+    return F.remap ? xcd_chunk(blockIdx.x, F.ncb8) : blockIdx.x;
 }
 
 template <class T>
@@ -217,7 +225,7 @@ __global__ void __launch_bounds__(kBlock) k_pre(FDev F, Consts<double> K, Consts
 // (the split variant; the merged variant forms them inside stage 1).
 template <class SD, class R>
 __global__ void __launch_bounds__(kBlock) k_dv(FDev F, Consts<double> K, Consts<R> KR) {
-    int i = logical_block(F.remap) * kBlock + threadIdx.x;
+    int i = cell_block(F) * kBlock + threadIdx.x;
     if (i >= F.n) return;
     const int nt = F.nt, n = F.n, m = F.m;
     const double *Q = F.qn;
@@ -324,7 +332,8 @@ template <class R, class SI, class SO, class SD, int STAGE, bool DV = true>
 __global__ void __launch_bounds__(kBlock) k_rk(FDev F, Consts<double> K, Consts<R> KR,
                                                double alpha) {
     constexpr bool kInlineDV = STAGE == 1 && DV;
-    const int i = logical_block(F.remap) * kBlock + threadIdx.x;
+    const int lb = cell_block(F);
+    const int i = lb * kBlock + threadIdx.x;
     const int nt = F.nt, n = F.n;
     const void *qin = STAGE == 1 ? (const void *)F.qn : (const void *)F.st[STAGE & 1];
     const size_t sin = STAGE == 1 ? (size_t)nt : (size_t)n;
@@ -522,7 +531,10 @@ __global__ void __launch_bounds__(kBlock) k_rk(FDev F, Consts<double> K, Consts<
             mx[q] = active ? fabs(d) : 0.0;
             bad |= active && !isfinite(cnew[q]);
         }
-        block_partials_write(sq, mx, bad, F.part + (size_t)blockIdx.x * kPartStride);
+        // slot = logical block: the same cells, summed in the same order, as the
+        // routine-level monitor's block partials
+        if (lb * kBlock < n)
+            block_partials_write(sq, mx, bad, F.part + (size_t)lb * kPartStride);
         if (blockIdx.x == 0 && threadIdx.x == 0) F.ctr[3] = 1;   // partials pending
     }
 }
@@ -565,7 +577,7 @@ hipError_t enqueue_iteration(tdt_ctx *c, FusedBufs &B, const Consts<R> &KR) {
         launch(c, B, 5, kq_stage<R, SS, double, SD, 4>, nq, F, c->K, KR, 1.0);
         return hipGetLastError();
     }
-    const unsigned nbc = B.nb_cells;
+    const unsigned nbc = B.F.ncb8;
     const int ncb8 = (B.nb_cells + 7) & ~7, nnb8 = (B.nb_nodes + 7) & ~7;
     launch(c, B, 0, k_pre<SD, R>, ncb8 + nnb8 + 1, F, c->K, KR, B.nb_cells, ncb8, nnb8);
     if (B.variant == kVariantSplit) {

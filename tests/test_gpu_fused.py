@@ -65,15 +65,18 @@ def test_fused_matches_reference(golden, case):
 
 
 @pytest.mark.parametrize("case", ["t2d_chan", "t2d_ramp", "taxi_duct", "t7_plate", "case1_axi"])
-def test_fused_equals_routine_path(golden, case):
+def test_fused_equals_routine_path(golden, case, monkeypatch):
     """The fused kernels and the routine-level kernels compute the same
-    expressions in the same order: the states must be bitwise identical."""
+    expressions in the same order: the states must be bitwise identical, and
+    so must the monitor norms (merged kernel set: one cell per lane)."""
     import tdt
+    monkeypatch.setenv("TDT_KERNELS", "merged")
     g = golden(case)
     k = 7
     with tdt.Solver(g.mesh, g.flow, "fp64") as a, tdt.Solver(g.mesh, g.flow, "fp64") as b:
         a.cold_start()
         b.cold_start()
+        hist = []
         for _ in range(k):
             a.timestep()
             a.dissipation()
@@ -85,10 +88,13 @@ def test_fused_equals_routine_path(golden, case):
                 a.source_terms()
                 a.residual()
                 a.update(ALPHAS[st])
-            a.monitor()
-        b.iterate(k)
+            hist.append(a.monitor())
+        l2, linf, _ = b.iterate(k)
         sa, sb = a.get_state(), b.get_state()
         assert np.array_equal(sa, sb), f"max diff {np.max(np.abs(sa - sb))}"
+        # the norms too: the same block partials, folded in the same order
+        assert np.array_equal(np.array([h[0] for h in hist]), l2)
+        assert np.array_equal(np.array([h[1] for h in hist]), linf)
         ne = g.mesh.neles
         for nm in ("D241", "D245", "VISU", "VIST", "VISR", "UN", "RN", "DT"):
             assert np.array_equal(a.get_array(nm)[:ne], b.get_array(nm)[:ne]), nm
