@@ -28,7 +28,7 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "2d-turbulent-navier-stokes_amd"))
 
 import tdt  # noqa: E402
-from tdt import caseio, gridgen  # noqa: E402
+from tdt import analysis, caseio, gridgen  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 DESCRIBE = {"channel_97x49": "2D turbulent channel / flat plate",
@@ -123,6 +123,25 @@ def run_gpu(mesh, flow, precision, steps, warmup, ordering="auto"):
     return elapsed, state, l2, bad, ktimes, kset
 
 
+def converged_comparison(mesh, flow, iters):
+    """FP64 and FP32-mixed run from the same cold start for `iters` iterations
+    (the case's NITER): residual drop of each and the FP32-mixed error against
+    FP64 in p, rho, u, nu~ and the wall shear stress."""
+    out = {"iterations": iters}
+    states = {}
+    for prec in ("fp64", "fp32-mixed"):
+        with tdt.Solver(mesh, flow, prec) as s:
+            s.cold_start()
+            l2, _, bad = s.iterate(iters)
+            states[prec] = s.get_state()
+            drop = l2[-1] / np.maximum(l2[:10].max(axis=0), 1e-300)
+            out[f"{prec}_l2_drop"] = [float(f"{v:.3e}") for v in drop]
+            out[f"{prec}_nonfinite_iter"] = int(bad)
+    err = analysis.compare(mesh, flow, states["fp32-mixed"], states["fp64"])
+    out["rel_l2_vs_fp64"] = {k: float(f"{v:.3e}") for k, v in err.items()}
+    return out
+
+
 def cpu_baseline(mesh, flow, target_s: float = 15.0):
     """The reference executable (built from its sources by oracle/build_ref.sh)
     on this host, one core; loop time = t(NITER=k) - t(NITER=0)."""
@@ -189,6 +208,9 @@ def main():
     ap.add_argument("--workload", default="duct_385x129", choices=sorted(gridgen.WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-mixed", action="store_true", help="skip the FP32-mixed run")
+    ap.add_argument("--converge", type=int, default=-1, metavar="ITERS",
+                    help="iterations of the converged FP32-mixed vs FP64 comparison "
+                         "(default: the case's NITER; 0 skips it)")
     ap.add_argument("--precision", default=None, choices=["fp64", "fp32-mixed"],
                     help="profile one precision only (prints no JSON line)")
     ap.add_argument("--ordering", default="auto",
@@ -244,6 +266,11 @@ def main():
     if mixed is not None:
         mixed["roofline"] = roofline(args.workload, "fp32-mixed", mesh, flow, kt32, ks32, el32,
                                      args.steps)
+        citers = flow.niter if args.converge < 0 else args.converge
+        if citers > 0:
+            mixed["converged"] = converged_comparison(mesh, flow, citers)
+            print(f"bench: converged comparison {mixed['converged']}", file=sys.stderr,
+                  flush=True)
 
     cpu = None if args.no_cpu else cpu_baseline(mesh, flow)
     line = {

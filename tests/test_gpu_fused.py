@@ -254,3 +254,23 @@ def test_fp32_geometry_copies_follow_set_array(golden):
             out[prec] = s.get_state()
     errs = field_errors(out["fp32-mixed"], out["fp64"], g.flow)
     assert errs["p"] < 1e-5 and errs["rho"] < 1e-5, errs
+
+
+def test_fp32_mixed_error_at_duct_run_length(monkeypatch):
+    """The FP32-mixed path against FP64 on the 385x129 duct after the case's
+    20000 iterations: pressure within 1e-5 relative L2 (the precision target),
+    density too; nu~ and the wall shear stress are reported by bench.py."""
+    import tdt
+    from tdt import analysis, gridgen
+    monkeypatch.setenv("TDT_KERNELS", "merged")
+    mesh, flow = gridgen.duct(), gridgen.flow_duct()
+    states = {}
+    for prec in ("fp64", "fp32-mixed"):
+        with tdt.Solver(mesh, flow, prec) as s:
+            s.cold_start()
+            _, _, bad = s.iterate(flow.niter, history=False)
+            assert bad == 0
+            states[prec] = s.get_state()
+    err = analysis.compare(mesh, flow, states["fp32-mixed"], states["fp64"])
+    print("fp32-mixed vs fp64 after", flow.niter, "iterations:", err)
+    assert err["p"] <= 1e-5 and err["rho"] <= 1e-5, err
