@@ -452,7 +452,7 @@ int tdt_kernel_times(tdt_ctx *c, double *times_ms) {
 int tdt_kernel_set(tdt_ctx *c, int32_t *set) {
     CHECK_CTX(c);
     if (!set) return TDT_ERR_ARG;
-    *set = tdt::fused_variant(c);
+    *set = tdt::fused_kernel_set(c);
     return TDT_OK;
 }
 

@@ -92,7 +92,7 @@ hipError_t launch_copy_cold(tdt_ctx *c);
 int fused_setup(tdt_ctx *c);
 void fused_free(tdt_ctx *c);
 int fused_iterate(tdt_ctx *c, int niter, double *l2, double *linf, int *first_nonfinite);
-int fused_variant(tdt_ctx *c);   // kernel set in use, -1 before the first iterate
+int fused_kernel_set(tdt_ctx *c);   // kernel set in use, -1 before the first iterate
 
 inline unsigned nblocks(int count) { return (unsigned)((count + kBlock - 1) / kBlock); }
 

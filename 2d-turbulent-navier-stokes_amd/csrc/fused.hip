@@ -111,13 +111,19 @@ int fused_setup(tdt_ctx *c) {
     B->nb_nodes = (int)nblocks(D.m);
     B->nbq_cells = (D.n + kBlock / 4 - 1) / (kBlock / 4);
     B->nbq_nodes = (D.m + kBlock / 4 - 1) / (kBlock / 4);
-    // TDT_KERNELS=merged|split|quad pins a kernel set; otherwise the first
-    // tdt_iterate call times them all (autotune)
+    // TDT_KERNELS=merged|split|quad (or a kernel-set number) pins the kernel
+    // set; otherwise the first tdt_iterate call picks it by timing (autotune)
     const char *kv = std::getenv("TDT_KERNELS");
     const std::string ks = kv ? kv : "";
     if (ks == "merged" || ks == "split" || ks == "quad") {
-        B->variant = ks == "merged" ? kVariantMerged : ks == "split" ? kVariantSplit : kVariantQuad;
+        B->kset = ks == "merged" ? kSetMerged : ks == "split" ? kSetSplit : kSetQuad;
         B->tuned = true;
+    } else if (!ks.empty() && ks.find_first_not_of("0123456789") == std::string::npos) {
+        const int v = std::atoi(ks.c_str());
+        if ((v & ~(kPreQuad | kSt1Mask | kStQuad)) == 0 && (v & kSt1Mask) != kSt1Mask) {
+            B->kset = v;
+            B->tuned = true;
+        }
     }
     // fused-only neighbour and node-gather tables (built from the prepared mesh)
     const PreparedMesh &pm = c->pm;
@@ -219,41 +225,87 @@ static void drop_graphs(FusedBufs &B) {
     if (B.graphK) { (void)hipGraphDestroy(B.graphK); B.graphK = nullptr; }
 }
 
-// Times each kernel set for a few iterations on the current state and keeps the
-// fastest.  Everything an iteration reads besides Q^n is recomputed within it,
-// so restoring Q^n after each trial leaves the solve exactly where it was.  In
-// FP64 the sets compute bitwise-identical states, so the choice is invisible.
+// Runs `iters` iterations of kernel set `kset` eagerly.  With per_slot, each
+// kernel is bracketed by events and per_slot[q] receives the average time of
+// slot q; otherwise *total_ms receives the whole run's time.
+static hipError_t time_set(tdt_ctx *c, FusedBufs &B, int kset, int iters, double *per_slot,
+                           float *total_ms) {
+    const FDev &F = B.F;
+    B.kset = kset;
+    c->profiling = per_slot ? 1 : 0;
+    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, c->stream, F.ctr, F.dtslot);
+    hipError_t e = enqueue(c, B);                                   // warm-up
+    if (e == hipSuccess && per_slot) {
+        e = hipEventSynchronize(B.ev[2 * kNumFusedKernels - 1]);
+        for (int q = 0; q < kNumFusedKernels; ++q) per_slot[q] = 0.0;
+        for (int it = 0; it < iters && e == hipSuccess; ++it) {
+            e = enqueue(c, B);
+            if (e == hipSuccess) e = hipEventSynchronize(B.ev[2 * kNumFusedKernels - 1]);
+            for (int q = 0; q < kNumFusedKernels && e == hipSuccess; ++q) {
+                float ms = 0.f;
+                e = hipEventElapsedTime(&ms, B.ev[2 * q], B.ev[2 * q + 1]);
+                per_slot[q] += ms / iters;
+            }
+        }
+    } else if (e == hipSuccess) {
+        e = hipEventRecord(B.ev[10], c->stream);
+        for (int r = 0; r < iters && e == hipSuccess; ++r) e = enqueue(c, B);
+        if (e == hipSuccess) e = hipEventRecord(B.ev[11], c->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(B.ev[11]);
+        if (e == hipSuccess) e = hipEventElapsedTime(total_ms, B.ev[10], B.ev[11]);
+    }
+    c->profiling = 0;
+    return e;
+}
+
+// Picks the kernel set by timing on the current state.  The uniform sets
+// (merged, split, quad) are timed kernel by kernel; the fastest choice for each
+// slot (pre-pass, stage 1, stages 2-4) makes a candidate, which then competes
+// with the uniform sets on whole iterations.  Everything an iteration reads
+// besides Q^n is recomputed within it, so restoring Q^n after each trial leaves
+// the solve exactly where it was; in FP64 every set computes the same bits.
 static hipError_t autotune(tdt_ctx *c, FusedBufs &B) {
     const FDev &F = B.F;
     const size_t qbytes = (size_t)5 * F.nt * sizeof(double);
-    hipError_t e = hipMemcpyAsync(B.qbak, F.qn, qbytes, hipMemcpyDeviceToDevice, c->stream);
     const int prof = c->profiling;
-    c->profiling = 0;
-    double best = 1e300;
-    int bestv = kVariantMerged;
-    for (int v = 0; v < kNumVariants && e == hipSuccess; ++v) {
-        B.variant = v;
-        hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, c->stream, F.ctr, F.dtslot);
-        e = enqueue(c, B);                                   // warm-up
-        if (e == hipSuccess) e = hipEventRecord(B.ev[10], c->stream);
-        for (int r = 0; r < kTuneIterations && e == hipSuccess; ++r) e = enqueue(c, B);
-        if (e == hipSuccess) e = hipEventRecord(B.ev[11], c->stream);
-        if (e == hipSuccess) e = hipEventSynchronize(B.ev[11]);
-        float ms = 0.f;
-        if (e == hipSuccess) e = hipEventElapsedTime(&ms, B.ev[10], B.ev[11]);
+    hipError_t e = hipMemcpyAsync(B.qbak, F.qn, qbytes, hipMemcpyDeviceToDevice, c->stream);
+    const int sets[3] = {kSetMerged, kSetSplit, kSetQuad};
+    double t[3][kNumFusedKernels] = {};
+    for (int k = 0; k < 3 && e == hipSuccess; ++k) {
+        e = time_set(c, B, sets[k], kTuneIterations, t[k], nullptr);
         if (e == hipSuccess)
             e = hipMemcpyAsync(F.qn, B.qbak, qbytes, hipMemcpyDeviceToDevice, c->stream);
-        if (e == hipSuccess && ms < best) { best = ms; bestv = v; }
+    }
+    // per slot: pre-pass (merged's k_pre vs quad's kq_pre), stage 1 (merged,
+    // split = k_dv + stage 1, quad), stages 2-4 (merged's vs quad's)
+    int comp = (t[2][0] < t[0][0]) ? kPreQuad : 0;
+    const double s1[3] = {t[0][2], t[1][1] + t[1][2], t[2][2]};
+    const int s1set[3] = {0, kSt1Split, kSt1Quad};
+    int b1 = 0;
+    for (int k = 1; k < 3; ++k) b1 = s1[k] < s1[b1] ? k : b1;
+    comp |= s1set[b1];
+    if (t[2][3] + t[2][4] + t[2][5] < t[0][3] + t[0][4] + t[0][5]) comp |= kStQuad;
+    // whole iterations: the composed set against the uniform ones
+    const int cand[4] = {kSetMerged, kSetSplit, kSetQuad, comp};
+    int best = kSetMerged;
+    float best_ms = 3.4e38f;
+    for (int k = 0; k < 4 && e == hipSuccess; ++k) {
+        if (k == 3 && (comp == kSetMerged || comp == kSetSplit || comp == kSetQuad)) break;
+        float ms = 0.f;
+        e = time_set(c, B, cand[k], kTuneIterations, nullptr, &ms);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(F.qn, B.qbak, qbytes, hipMemcpyDeviceToDevice, c->stream);
+        if (e == hipSuccess && ms < best_ms) { best_ms = ms; best = cand[k]; }
     }
     c->profiling = prof;
-    B.variant = bestv;
+    B.kset = best;
     B.tuned = true;
     drop_graphs(B);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     return e;
 }
 
-int fused_variant(tdt_ctx *c) { return c->fused ? c->fused->variant : -1; }
+int fused_kernel_set(tdt_ctx *c) { return c->fused ? c->fused->kset : -1; }
 
 int fused_iterate(tdt_ctx *c, int niter, double *l2, double *linf, int *first_nonfinite) {
     int rc = fused_setup(c);
@@ -314,7 +366,7 @@ int fused_iterate(tdt_ctx *c, int niter, double *l2, double *linf, int *first_no
             }
         }
         for (int q = 0; q < kNumFusedKernels; ++q) c->ktimes[q] /= (niter > 0 ? niter : 1);
-        if (B.variant != kVariantSplit) c->ktimes[1] = 0.0;   // no separate k_dv launch
+        if ((B.kset & kSt1Mask) != kSt1Split) c->ktimes[1] = 0.0;   // no k_dv launch
     } else if (e == hipSuccess) {
         if (!B.exec1) e = build_graph(c, B, 1, &B.graph1, &B.exec1);
         if (e == hipSuccess && !B.execK && niter >= B.chunk)
@@ -331,7 +383,7 @@ int fused_iterate(tdt_ctx *c, int niter, double *l2, double *linf, int *first_no
     }
     if (e == hipSuccess) {
         hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, c->stream, F,
-                           B.variant == kVariantQuad ? B.nbq_cells : B.nb_cells);
+                           partial_blocks(B, B.kset));
         hipLaunchKernelGGL(k_cold_from_qn, dim3(nblocks(F.n)), dim3(kBlock), 0, c->stream, F,
                            c->D.cold);
     }

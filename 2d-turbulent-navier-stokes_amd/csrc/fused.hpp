@@ -67,18 +67,28 @@ struct FusedBufs {
     int chunk = 8;
     hipEvent_t ev[12];
     bool have_events = false;
-    int variant = 0;                 // kVariant*: kernel set of the iteration
-    bool tuned = false;              // variant picked by timing (autotune)
+    int kset = 0;                    // kernel set of the iteration (kPre* | kSt1* | kSt*)
+    bool tuned = false;              // kernel set picked by timing (autotune)
     double *qbak = nullptr;          // Q^n saved while candidates are timed
     int nbq_cells = 0, nbq_nodes = 0;
     int geom_version = -1;           // tdt_ctx::geom_version the FP32 copies match
 };
 
-// Kernel sets of one iteration; all give bitwise-identical results.
-constexpr int kVariantMerged = 0;  // k_pre, k_rk x4 (stage 1 forms D and VIS)
-constexpr int kVariantSplit = 1;   // k_pre, k_dv, k_rk x4
-constexpr int kVariantQuad = 2;    // kq_pre, kq_stage x4 (four lanes per cell)
-constexpr int kNumVariants = 3;
+// Kernel set of one iteration, chosen per slot; every choice computes the same
+// values (FP64: bitwise).  Bit 0: pre-pass; bits 1-2: stage 1; bit 3: stages 2-4.
+constexpr int kPreQuad = 1;        // kq_pre instead of k_pre
+constexpr int kSt1Split = 1 << 1;  // k_dv + k_rk<1, DV=false> instead of k_rk<1> (merged)
+constexpr int kSt1Quad = 2 << 1;   // kq_stage<1>
+constexpr int kSt1Mask = 3 << 1;
+constexpr int kStQuad = 1 << 3;    // kq_stage<2..4> instead of k_rk<2..4>
+// the three uniform sets
+constexpr int kSetMerged = 0;
+constexpr int kSetSplit = kSt1Split;
+constexpr int kSetQuad = kPreQuad | kSt1Quad | kStQuad;
+// stage 4 writes the monitor partials: one per block of its kind
+inline int partial_blocks(const FusedBufs &B, int kset) {
+    return (kset & kStQuad) ? B.nbq_cells : B.nb_cells;
+}
 // trial iterations per kernel set when the first tdt_iterate call picks one
 constexpr int kTuneIterations = 4;
 

@@ -104,15 +104,15 @@ __device__ __forceinline__ void node_gather_list(const FDev &F, const Consts<R> 
 // iteration's monitor partials (off every cell's path).
 template <class SD, class R>
 __global__ void __launch_bounds__(kBlock) k_pre(FDev F, Consts<double> K, Consts<R> KR,
-                                                int nb_cells, int ncb8, int nnb8) {
+                                                int nb_partials, int ncb8, int nnb8) {
     int b = blockIdx.x;
     if (b == (int)gridDim.x - 1) {
-        fold_monitor(F, nb_cells);
+        fold_monitor(F, nb_partials);
         return;
     }
     if (b < ncb8) {
         const int cb = F.remap ? xcd_chunk(b, ncb8) : b;
-        if (cb >= nb_cells) return;
+        if (cb * kBlock >= F.n) return;
         int i = cb * kBlock + threadIdx.x;
         double dtl = 100000.0;
         if (i < F.n) {
@@ -563,36 +563,43 @@ void launch(tdt_ctx *c, FusedBufs &B, int slot, K kern, unsigned grid, Args... a
 template <class R, class SS, class SD>
 hipError_t enqueue_iteration(tdt_ctx *c, FusedBufs &B, const Consts<R> &KR) {
     const FDev &F = B.F;
-    if (B.variant == kVariantQuad) {
-        const unsigned nq = B.nbq_cells;
-        launch(c, B, 0, kq_pre<SD, R>, nq + B.nbq_nodes + 1, F, c->K, KR, B.nbq_cells,
-               B.nbq_cells);
+    const int ks = B.kset;
+    const unsigned nbc = F.ncb8, nq = B.nbq_cells;
+    const int nparts = partial_blocks(B, ks);
+    auto no_dv_slot = [&]() {   // slot 1 unused: dissipation and viscous live in stage 1
         if (c->profiling && B.have_events) {
             (void)hipEventRecord(B.ev[2], c->stream);
             (void)hipEventRecord(B.ev[3], c->stream);
         }
+    };
+    if (ks & kPreQuad) {
+        launch(c, B, 0, kq_pre<SD, R>, nq + B.nbq_nodes + 1, F, c->K, KR, B.nbq_cells, nparts);
+    } else {
+        const int nnb8 = (B.nb_nodes + 7) & ~7;
+        launch(c, B, 0, k_pre<SD, R>, F.ncb8 + nnb8 + 1, F, c->K, KR, nparts, F.ncb8, nnb8);
+    }
+    switch (ks & kSt1Mask) {
+    case kSt1Split:
+        launch(c, B, 1, k_dv<SD, R>, nbc, F, c->K, KR);
+        launch(c, B, 2, k_rk<R, double, SS, SD, 1, false>, nbc, F, c->K, KR, 0.25);
+        break;
+    case kSt1Quad:
+        no_dv_slot();
         launch(c, B, 2, kq_stage<R, double, SS, SD, 1>, nq, F, c->K, KR, 0.25);
+        break;
+    default:
+        no_dv_slot();
+        launch(c, B, 2, k_rk<R, double, SS, SD, 1>, nbc, F, c->K, KR, 0.25);
+    }
+    if (ks & kStQuad) {
         launch(c, B, 3, kq_stage<R, SS, SS, SD, 2>, nq, F, c->K, KR, 1.0 / 3.0);
         launch(c, B, 4, kq_stage<R, SS, SS, SD, 3>, nq, F, c->K, KR, 0.5);
         launch(c, B, 5, kq_stage<R, SS, double, SD, 4>, nq, F, c->K, KR, 1.0);
-        return hipGetLastError();
-    }
-    const unsigned nbc = B.F.ncb8;
-    const int ncb8 = (B.nb_cells + 7) & ~7, nnb8 = (B.nb_nodes + 7) & ~7;
-    launch(c, B, 0, k_pre<SD, R>, ncb8 + nnb8 + 1, F, c->K, KR, B.nb_cells, ncb8, nnb8);
-    if (B.variant == kVariantSplit) {
-        launch(c, B, 1, k_dv<SD, R>, nbc, F, c->K, KR);
-        launch(c, B, 2, k_rk<R, double, SS, SD, 1, false>, nbc, F, c->K, KR, 0.25);
     } else {
-        if (c->profiling && B.have_events) {   // dissipation and viscous live in stage 1
-            (void)hipEventRecord(B.ev[2], c->stream);
-            (void)hipEventRecord(B.ev[3], c->stream);
-        }
-        launch(c, B, 2, k_rk<R, double, SS, SD, 1>, nbc, F, c->K, KR, 0.25);
+        launch(c, B, 3, k_rk<R, SS, SS, SD, 2>, nbc, F, c->K, KR, 1.0 / 3.0);
+        launch(c, B, 4, k_rk<R, SS, SS, SD, 3>, nbc, F, c->K, KR, 0.5);
+        launch(c, B, 5, k_rk<R, SS, double, SD, 4>, nbc, F, c->K, KR, 1.0);
     }
-    launch(c, B, 3, k_rk<R, SS, SS, SD, 2>, nbc, F, c->K, KR, 1.0 / 3.0);
-    launch(c, B, 4, k_rk<R, SS, SS, SD, 3>, nbc, F, c->K, KR, 0.5);
-    launch(c, B, 5, k_rk<R, SS, double, SD, 4>, nbc, F, c->K, KR, 1.0);
     return hipGetLastError();
 }
 

@@ -238,12 +238,22 @@ class Solver:
         return out
 
     KERNEL_SETS = ("merged", "split", "quad")
+    _UNIFORM = {0: "merged", 2: "split", 13: "quad"}
 
     def kernel_set(self) -> str:
-        """Name of the fused kernel set in use ("" before the first tdt_iterate)."""
+        """The fused kernel set in use: "merged", "split", "quad", or for a mixed
+        choice e.g. "pre=quad,stage1=merged,stages=lane" ("" before the first
+        tdt_iterate)."""
         out = np.zeros(1, dtype=np.int32)
         self._check(self.L.tdt_kernel_set(self.h, _i(out)))
-        return self.KERNEL_SETS[out[0]] if out[0] >= 0 else ""
+        v = int(out[0])
+        if v < 0:
+            return ""
+        if v in self._UNIFORM:
+            return self._UNIFORM[v]
+        st1 = {0: "merged", 2: "split", 4: "quad"}[v & 6]
+        return (f"pre={'quad' if v & 1 else 'lane'},stage1={st1},"
+                f"stages={'quad' if v & 8 else 'lane'}")
 
     def synchronize(self):
         self._check(self.L.tdt_synchronize(self.h))

@@ -132,16 +132,22 @@ int tdt_set_array(tdt_ctx *ctx, const char *name, const double *in, int32_t len)
  * stage 1, stage 2, stage 3, stage 4. */
 int tdt_set_profiling(tdt_ctx *ctx, int32_t enable);
 int tdt_kernel_times(tdt_ctx *ctx, double *times_ms);
-/* Kernel set the fused path uses (TDT_KERNELS_*), or -1 before the first
- * tdt_iterate.  In FP64 the sets compute bitwise identical states (the quad set
- * sums the L2 norms in another order); in FP32-mixed they agree to FP32
- * rounding.  The first tdt_iterate call
- * times a few iterations of each on the current state (restoring it after each
- * trial) and keeps the fastest, unless the environment variable
- * TDT_KERNELS=merged|split|quad, read by the first tdt_iterate, pins one. */
-#define TDT_KERNELS_MERGED 0 /* stage 1 computes dissipation and viscous fluxes */
-#define TDT_KERNELS_SPLIT 1  /* separate dissipation+viscous kernel before stage 1 */
-#define TDT_KERNELS_QUAD 2   /* four lanes per cell, one per face */
+/* Kernel set the fused path uses, or -1 before the first tdt_iterate: a choice
+ * per kernel slot, as bits.  Every choice computes the same values (FP64: the
+ * same bits, except that stages 2-4 in quad form sum the L2 norms in another
+ * order; FP32-mixed: the same to FP32 rounding).  The first tdt_iterate call
+ * times the uniform sets kernel by kernel on the current state (restoring it
+ * after each trial), composes the fastest choice per slot and keeps whichever
+ * of that and the uniform sets runs an iteration fastest, unless the
+ * environment variable TDT_KERNELS (merged, split, quad or a set number), read
+ * by the first tdt_iterate, pins one. */
+#define TDT_KERNELS_PRE_QUAD 1        /* pre-pass: four lanes per cell (else one)     */
+#define TDT_KERNELS_STAGE1_SPLIT 2    /* dissipation + viscous in a kernel of their own */
+#define TDT_KERNELS_STAGE1_QUAD 4     /* stage 1 (with D and VIS): four lanes per cell */
+#define TDT_KERNELS_STAGES_QUAD 8     /* stages 2-4: four lanes per cell               */
+#define TDT_KERNELS_MERGED 0          /* uniform sets: one lane per cell, D/VIS in stage 1 */
+#define TDT_KERNELS_SPLIT 2
+#define TDT_KERNELS_QUAD 13
 int tdt_kernel_set(tdt_ctx *ctx, int32_t *set);
 int tdt_synchronize(tdt_ctx *ctx);
 
