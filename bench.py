@@ -187,6 +187,60 @@ def cpu_baseline_port(mesh, flow, target_s):
             "kind": "port", "sample": f"C oracle (FP64 restatement), {k} iterations, {loop:.2f} s"}
 
 
+def stream_copy_gbs(mib: int = 1024, reps: int = 20) -> float:
+    """Achieved bandwidth of a plain device copy (read + write bytes / time):
+    the practical HBM bound on this box, beside the 8 TB/s peak.  Runs in a
+    process of its own: torch brings its own HIP runtime, which must not share
+    a process with the one libtdt uses."""
+    out = subprocess.run([sys.executable, os.path.abspath(__file__), "--stream-probe", str(mib),
+                          "--stream-reps", str(reps)], capture_output=True, text=True,
+                         timeout=300)
+    if out.returncode != 0:
+        raise RuntimeError(out.stderr.strip().splitlines()[-1] if out.stderr else "failed")
+    return float(out.stdout.strip().splitlines()[-1])
+
+
+def stream_probe(mib: int, reps: int) -> float:
+    import torch
+    n = mib * 1024 * 1024 // 8
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        y.copy_(x)
+    b.record()
+    torch.cuda.synchronize()
+    sec = a.elapsed_time(b) * 1e-3 / reps
+    del x, y
+    torch.cuda.empty_cache()
+    return 2 * n * 8 / sec / 1e9
+
+
+def large_case(stream_gbs, steps=200, warmup=50):
+    """The 2049x513 compression ramp (1.05 M cells: enough wavefronts to fill
+    the GPU), FP64 and FP32-mixed: throughput and the roofline of the dominant
+    kernel of each, for the HBM-bound regime the 385x129 duct cannot reach."""
+    mk_mesh, mk_flow = gridgen.WORKLOADS["ramp_2049x513"]
+    mesh, flow = mk_mesh(), mk_flow()
+    out = {"workload": f"ramp_2049x513: {DESCRIBE['ramp_2049x513']}, {mesh.neles} cells",
+           "steps": steps, "warmup": warmup}
+    for prec in ("fp64", "fp32-mixed"):
+        el, _, _, bad, kt, ks = run_gpu(mesh, flow, prec, steps, warmup)
+        v = mesh.neles * steps / el / 1e6
+        roof = roofline("ramp_2049x513", prec, mesh, flow, kt, ks, el, steps)
+        if stream_gbs:
+            roof["frac_of_stream"] = round(roof["achieved"] / stream_gbs, 4)
+        out[prec] = {"value": round(v, 2), "unit": "Mcell-iters/s", "kernel_set": ks,
+                     "kernel_ms": [round(float(x), 5) for x in kt], "nonfinite_iter": int(bad),
+                     "roofline": roof}
+        print(f"bench: ramp {prec} {v:.1f} Mcell-iters/s kernel set {ks} kernels(ms) "
+              f"{np.round(kt, 5).tolist()}", file=sys.stderr, flush=True)
+    return out
+
+
 def copy_probe(mib: int) -> None:
     """A plain elementwise copy of known size: y = x * 1 over MIB MiB of float64, three
     times.  Its FETCH_SIZE / WRITE_SIZE calibrate the HBM counters (expect MIB MiB each)."""
@@ -208,6 +262,12 @@ def main():
     ap.add_argument("--workload", default="duct_385x129", choices=sorted(gridgen.WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-mixed", action="store_true", help="skip the FP32-mixed run")
+    ap.add_argument("--no-stream", action="store_true", help="skip the copy-bandwidth probe")
+    ap.add_argument("--stream-probe", type=int, default=0, metavar="MIB",
+                    help="print the bandwidth of a plain MIB MiB device copy (GB/s) and exit")
+    ap.add_argument("--stream-reps", type=int, default=20, help=argparse.SUPPRESS)
+    ap.add_argument("--no-large", action="store_true",
+                    help="skip the 2049x513 ramp roofline run beside the main workload")
     ap.add_argument("--converge", type=int, default=-1, metavar="ITERS",
                     help="iterations of the converged FP32-mixed vs FP64 comparison "
                          "(default: the case's NITER; 0 skips it)")
@@ -220,6 +280,9 @@ def main():
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe(args.copy_probe)
+        return
+    if args.stream_probe:
+        print(f"{stream_probe(args.stream_probe, args.stream_reps):.1f}", flush=True)
         return
     if args.precision:
         mk_mesh, mk_flow = gridgen.WORKLOADS[args.workload]
@@ -272,6 +335,16 @@ def main():
             print(f"bench: converged comparison {mixed['converged']}", file=sys.stderr,
                   flush=True)
 
+    stream = None
+    if not args.no_stream:
+        try:
+            stream = round(stream_copy_gbs(), 1)
+        except Exception as exc:                  # torch missing or no device memory
+            print(f"bench: stream copy not measured: {exc}", file=sys.stderr, flush=True)
+    roof["stream_GBs"] = stream
+    if stream:
+        roof["frac_of_stream"] = round(roof["achieved"] / stream, 4)
+    large = None if args.no_large or args.workload == "ramp_2049x513" else large_case(stream)
     cpu = None if args.no_cpu else cpu_baseline(mesh, flow)
     line = {
         "metric": "Mcell-iters/s on 1 MI355X, FP64 vs FP32-mixed at fixed error; % HBM peak",
@@ -285,6 +358,7 @@ def main():
                    "precision": "fp64", "kernel_set": ks64, "nonfinite_iter": int(bad64)},
         "fp32_mixed": mixed,
         "roofline": roof,
+        "large_case": large,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
