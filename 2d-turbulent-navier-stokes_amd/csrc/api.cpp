@@ -364,6 +364,26 @@ int tdt_monitor(tdt_ctx *c, double *l2, double *linf) {
     return TDT_OK;
 }
 
+int tdt_postprocess(tdt_ctx *c, double *nodal) {
+    CHECK_CTX(c);
+    if (!nodal) return TDT_ERR_ARG;
+    const tdt::PreparedMesh &pm = c->pm;
+    if (!c->post && !(c->post = dev_alloc<double>(c, (size_t)6 * pm.m, false))) {
+        c->err = "postprocess: device allocation failed";
+        return TDT_ERR_DEVICE;
+    }
+    HIPCALL(c, tdt::launch_postprocess(c, c->post), "postprocess");
+    std::vector<double> h((size_t)6 * pm.m);
+    HIPCALL(c, hipMemcpyAsync(h.data(), c->post, h.size() * sizeof(double), hipMemcpyDeviceToHost,
+                              c->stream), "postprocess");
+    int rc = sync(c);
+    if (rc) return rc;
+    for (int k = 0; k < 6; ++k)
+        for (int q = 0; q < pm.m; ++q)
+            nodal[(size_t)k * pm.m + pm.node_new2old[q]] = h[(size_t)k * pm.m + q];
+    return TDT_OK;
+}
+
 int tdt_iterate(tdt_ctx *c, int32_t niter, double *l2, double *linf, int32_t *first_nonfinite) {
     CHECK_CTX(c);
     if (niter < 0) return TDT_ERR_ARG;

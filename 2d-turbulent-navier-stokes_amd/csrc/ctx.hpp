@@ -68,6 +68,7 @@ struct tdt_ctx {
     int profiling = 0;
     double ktimes[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     int geom_version = 0;     // bumped when tdt_set_array changes geometry
+    double *post = nullptr;   // [6][m] POSTPROCESS output, allocated on first use
 };
 
 namespace tdt {
@@ -87,6 +88,8 @@ hipError_t launch_monitor(tdt_ctx *c, int save_old, double *out);
 hipError_t launch_cold_start(tdt_ctx *c, double rho, double ru, double rv, double re,
                              double rr);
 hipError_t launch_copy_cold(tdt_ctx *c);
+// tec_plot.dat node columns U, V, P [bar], RHO, MACH, T into out[6][m].
+hipError_t launch_postprocess(tdt_ctx *c, double *out);
 
 // Fused path (fused.hip).
 int fused_setup(tdt_ctx *c);

@@ -21,6 +21,7 @@ constexpr double kCflTurb = 0.5;
 constexpr double kDiv23 = 2.0 / 3.0;
 constexpr double kYaxisThresh = 0.001, kAmuThresh = 0.001;
 constexpr double kBlExponent = 1.0 / 7.0;
+constexpr double kPaToBar = 0.00001;
 
 // Flow constants derived once on the host with the reference's expressions.
 template <class R>

@@ -264,6 +264,34 @@ __global__ void __launch_bounds__(kBlock) k_nodeavg(Dev D, Consts<double> K) {
     }
 }
 
+// POSTPROCESS (solver_state.f90:230-283): node averages of P, U, V, T over the
+// cells around each node and the ghosts on its faces, in the reference's order
+// (the node lists: cells by id, then ghosts in bc.in order; no overrides), then
+// the tec_plot.dat columns U, V, P [bar], RHO, MACH, T.  out: [6][m].
+__global__ void __launch_bounds__(kBlock) k_postprocess(Dev D, Consts<double> K, double *out) {
+    int v = gid();
+    if (v < D.m) {
+        const int j0 = D.nptr[v], j1 = D.nptr[v + 1];
+        double pn = 0.0, un = 0.0, vn = 0.0, tn = 0.0;
+        for (int j = j0; j < j1; ++j) {
+            const int id = D.nlist[j];
+            pn = pn + D.p[id];
+            un = un + D.u[id];
+            vn = vn + D.v[id];
+            tn = tn + D.t[id];
+        }
+        const double df = (double)(j1 - j0);
+        pn = pn / df; un = un / df; vn = vn / df; tn = tn / df;
+        const int m = D.m;
+        out[v] = un;
+        out[m + v] = vn;
+        out[2 * m + v] = pn * kPaToBar;
+        out[3 * m + v] = pn / (K.rr * tn);
+        out[4 * m + v] = sqrt(un * un + vn * vn) / sqrt(K.gamma_rr * tn);
+        out[5 * m + v] = tn;
+    }
+}
+
 // viscous.f90:62-70, 130-217
 __global__ void __launch_bounds__(kBlock) k_viscous(Dev D, Consts<double> K) {
     int i = gid();
@@ -450,6 +478,11 @@ hipError_t launch_monitor(tdt_ctx *c, int save_old, double *out) {
 
 hipError_t launch_cold_start(tdt_ctx *c, double r0, double r1, double r2, double r3, double r4) {
     LAUNCH(k_cold_start, c->D.n, c->D, r0, r1, r2, r3, r4);
+    return hipSuccess;
+}
+
+hipError_t launch_postprocess(tdt_ctx *c, double *out) {
+    LAUNCH(k_postprocess, c->D.m, c->D, c->K, out);
     return hipSuccess;
 }
 

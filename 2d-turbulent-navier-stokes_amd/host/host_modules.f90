@@ -54,46 +54,21 @@ CONTAINS
     ! POSTPROCESS (solver_state.f90:230-283): cell-to-node averages of P, U, V, T
     ! over the cells and boundary ghosts around each node, written as tec_plot.dat.
     SUBROUTINE POSTPROCESS
-        DOUBLE PRECISION, ALLOCATABLE :: PN(:), UN(:), VN(:), TN(:)
-        INTEGER, ALLOCATABLE :: ITEST(:)
-        INTEGER :: I, J, K, NG, NP, NS, N1, N2, IE
-        DOUBLE PRECISION :: PNN, UNN, VNN, TNN, AMN, RON, DF
+        ! solver_state.f90:230-283: the node averages and derived columns are
+        ! computed on the device (TDT_POSTPROCESS, same order and expressions);
+        ! the host writes tec_plot.dat in the reference's format
+        DOUBLE PRECISION, ALLOCATABLE :: NODAL(:, :)
+        INTEGER :: I, IE
         PRINT*, 'Postprocessing.'
         PRINT*, ' '
-        ALLOCATE(PN(NODS), UN(NODS), VN(NODS), TN(NODS), ITEST(NODS))
-        IF (.NOT. ALLOCATED(U)) ALLOCATE(U(NMAX), V(NMAX), P(NMAX), T(NMAX))
-        CALL FETCH('P', P, NMAX); CALL FETCH('U', U, NMAX)
-        CALL FETCH('V', V, NMAX); CALL FETCH('T', T, NMAX)
-        PN = 0.0D0; UN = 0.0D0; VN = 0.0D0; TN = 0.0D0; ITEST = 0
-        DO I = 1, NELES
-            DO J = 1, 4
-                K = NOD(I, J)
-                ITEST(K) = ITEST(K) + 1
-                PN(K) = PN(K) + P(I); UN(K) = UN(K) + U(I)
-                VN(K) = VN(K) + V(I); TN(K) = TN(K) + T(I)
-            END DO
-        END DO
-        DO I = 1, NGHOSTS
-            NG = NGHOST(I); NP = NPARENT(I); NS = NSIDE(I)
-            N1 = NOD(NP, NS); N2 = NOD(NP, MOD(NS, 4) + 1)
-            PN(N1) = PN(N1) + P(NG); UN(N1) = UN(N1) + U(NG)
-            VN(N1) = VN(N1) + V(NG); TN(N1) = TN(N1) + T(NG)
-            PN(N2) = PN(N2) + P(NG); UN(N2) = UN(N2) + U(NG)
-            VN(N2) = VN(N2) + V(NG); TN(N2) = TN(N2) + T(NG)
-            ITEST(N1) = ITEST(N1) + 1; ITEST(N2) = ITEST(N2) + 1
-        END DO
-        DO I = 1, NODS
-            DF = DBLE(ITEST(I))
-            PN(I) = PN(I)/DF; UN(I) = UN(I)/DF; VN(I) = VN(I)/DF; TN(I) = TN(I)/DF
-        END DO
+        ALLOCATE(NODAL(NODS, 6))
+        CALL TDT_CHECK(TDT_POSTPROCESS(CTX, NODAL), CTX, 'POSTPROCESS')
         OPEN(UNIT=26, FILE='tec_plot.dat')
         WRITE(26,*)'VARIABLES = "X", "Y", "U", "V", "P", "RHO", "MACH", "T"'
         WRITE(26,*)'ZONE F = FEPOINT, ET = QUADRILATERAL, N = ',NODS,', E = ',NELES
         DO I = 1, NODS
-            PNN = PN(I); UNN = UN(I); VNN = VN(I); TNN = TN(I)
-            AMN = DSQRT(UNN*UNN+VNN*VNN)/DSQRT(GAMA*RR*TNN)
-            RON = PNN/(RR*TNN)
-            WRITE(26,*) X(I), Y(I), UNN, VNN, PNN*PA_TO_BAR, RON, AMN, TNN
+            WRITE(26,*) X(I), Y(I), NODAL(I,1), NODAL(I,2), NODAL(I,3), NODAL(I,4), &
+                        NODAL(I,5), NODAL(I,6)
         END DO
         DO IE = 1, NELES
             WRITE(26,*) NOD(IE,1), NOD(IE,2), NOD(IE,3), NOD(IE,4)

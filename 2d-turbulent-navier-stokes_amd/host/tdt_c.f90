@@ -106,6 +106,12 @@ MODULE TDT_C
             REAL(C_DOUBLE), INTENT(OUT) :: OUT(*)
             INTEGER(C_INT32_T), VALUE :: LEN
         END FUNCTION
+        ! POSTPROCESS node values: NODAL(NODS,6) = U, V, P [bar], RHO, MACH, T
+        INTEGER(C_INT32_T) FUNCTION TDT_POSTPROCESS(CTX, NODAL) BIND(C, NAME='tdt_postprocess')
+            IMPORT :: C_PTR, C_INT32_T, C_DOUBLE
+            TYPE(C_PTR), VALUE :: CTX
+            REAL(C_DOUBLE), INTENT(OUT) :: NODAL(*)
+        END FUNCTION
     END INTERFACE
 
 CONTAINS

@@ -88,6 +88,7 @@ def load(path: str = LIB_PATH):
     L.tdt_set_profiling.argtypes = [vp, ctypes.c_int32]
     L.tdt_kernel_times.argtypes = [vp, _dp]
     L.tdt_kernel_set.argtypes = [vp, _ip]
+    L.tdt_postprocess.argtypes = [vp, _dp]
     _lib = L
     return L
 
@@ -192,6 +193,14 @@ class Solver:
         linf = np.zeros(5)
         self._check(self.L.tdt_monitor(self.h, _d(l2), _d(linf)))
         return l2, linf
+
+    def postprocess(self) -> dict:
+        """POSTPROCESS (solver_state.f90:230-283) on the device: node values of the
+        tec_plot.dat columns U, V, P [bar], RHO, MACH, T from the current P, U, V, T
+        arrays, in grid.in node order."""
+        out = np.empty((6, self.nods))
+        self._check(self.L.tdt_postprocess(self.h, _d(out)))
+        return dict(zip(("U", "V", "P_bar", "RHO", "MACH", "T"), out))
 
     # ---- fused hot path ----------------------------------------------------------------
     def iterate(self, niter: int, history: bool = True, check: bool = True):

@@ -105,6 +105,13 @@ int tdt_update(tdt_ctx *ctx, double alpha); /* UPDATE_SOLUTION(ALPHA) residual.f
  * TDT_ERR_NONFINITE if the state holds NaN/Inf (the reference masks it). */
 int tdt_monitor(tdt_ctx *ctx, double *l2, double *linf);
 
+/* POSTPROCESS (solver_state.f90:230-283): from the P, U, V, T arrays as
+ * COMPUTE_PRIMITIVE_VARS and APPLY_BOUNDARY leave them, the node averages over
+ * the cells around each node and the ghosts on its faces, in the reference's
+ * summation order, and the tec_plot.dat columns derived from them.
+ * nodal: [6][nods] = U, V, P [bar], RHO, MACH, T, in grid.in node order. */
+int tdt_postprocess(tdt_ctx *ctx, double *nodal);
+
 /* ---- fused hot path ------------------------------------------------------------- */
 /* niter full RK4 iterations (turbnstwod.f90:95-168) on the device: one pre-pass
  * (time step, dissipation sensors, nodal averages) and one fused kernel per RK

@@ -146,3 +146,45 @@ def test_routine_chain_without_injection(golden, case):
         got = s.get_state()
         for m in range(5):
             close(f"C{m + 1}", got[m], g["final"][m], tol=1e-12)
+
+
+def _postprocess_restated(mesh, flow, p, u, v, t):
+    """solver_state.f90:230-283 in numpy: the same contributions in the same order."""
+    rr = 8314.0 / flow.amol
+    nn = mesh.nods
+    acc = np.zeros((4, nn))
+    cnt = np.zeros(nn)
+    vals = np.stack([p, u, v, t])
+    for i in range(mesh.neles):                   # cells by id, their four nodes
+        for j in range(4):
+            k = mesh.nod[j, i] - 1
+            acc[:, k] = acc[:, k] + vals[:, i]
+            cnt[k] += 1
+    for g in range(mesh.nghosts):                 # then the ghosts in bc.in order
+        ng, npar, ns = mesh.ghost[g] - 1, mesh.parent[g] - 1, mesh.side[g]
+        n1, n2 = mesh.nod[ns - 1, npar] - 1, mesh.nod[ns % 4, npar] - 1
+        for k in (n1, n2):
+            acc[:, k] = acc[:, k] + vals[:, ng]
+            cnt[k] += 1
+    pn, un, vn, tn = acc / cnt
+    return {"U": un, "V": vn, "P_bar": pn * 0.00001, "RHO": pn / (rr * tn),
+            "MACH": np.sqrt(un * un + vn * vn) / np.sqrt(flow.gama * rr * tn), "T": tn}
+
+
+@pytest.mark.parametrize("case", ["case1", "taxi_duct"])
+def test_postprocess_matches_restatement(golden, case):
+    """tdt_postprocess reproduces POSTPROCESS's node values bit for bit from the
+    P, U, V, T arrays the routines leave (here after a few fused iterations and
+    one routine-level stage)."""
+    import tdt
+    g = golden(case)
+    with tdt.Solver(g.mesh, g.flow, "fp64") as s:
+        s.cold_start()
+        s.iterate(5)
+        s.primitives()
+        s.apply_boundary()
+        got = s.postprocess()
+        arrs = {k: s.get_array(k) for k in ("P", "U", "V", "T")}
+    want = _postprocess_restated(g.mesh, g.flow, arrs["P"], arrs["U"], arrs["V"], arrs["T"])
+    for k in want:
+        assert np.array_equal(got[k], want[k]), (k, np.max(np.abs(got[k] - want[k])))
