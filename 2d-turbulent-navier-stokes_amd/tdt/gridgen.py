@@ -135,14 +135,38 @@ def cosine_duct_radius(x: float, r_in=50.0, r_throat=25.0, x_throat=150.0, r_exi
     return r_throat + (r_exit - r_throat) * 0.5 * (1.0 - math.cos(math.pi * s))
 
 
+def hicks_henne(x: float, amplitudes, centres=None, length: float = 400.0,
+                width: float = 3.0) -> float:
+    """Sum of Hicks-Henne bumps a_k sin^width(pi t^(ln 0.5 / ln c_k)), t = x / length,
+    each peaking (value a_k) at x = centre_k; centres default to len(amplitudes)
+    stations evenly spaced over 50..350 mm.  Zero at both ends."""
+    amplitudes = list(amplitudes)
+    if not amplitudes:
+        return 0.0
+    if centres is None:
+        k = len(amplitudes)
+        centres = [50.0 + 300.0 * i / (k - 1) for i in range(k)] if k > 1 else [200.0]
+    t = min(max(x / length, 0.0), 1.0)
+    if t <= 0.0 or t >= 1.0:
+        return 0.0
+    out = 0.0
+    for a, c in zip(amplitudes, centres):
+        e = math.log(0.5) / math.log(c / length)
+        out += a * math.sin(math.pi * t ** e) ** width
+    return out
+
+
 def duct(nx=385, ny=129, h1=0.005, jitter=0.01, seed=DEFAULT_SEED, bc=None,
-         x_exit=400.0, radius=None) -> Mesh:
+         x_exit=400.0, radius=None, bumps=None) -> Mesh:
     """Axisymmetric converging-diverging duct; axis at y=0, wall on top.
 
-    Radial stretching starts from a first cell h1 (mm) at the wall.
+    Radial stretching starts from a first cell h1 (mm) at the wall.  bumps:
+    Hicks-Henne amplitudes (mm) added to the wall radius (centres over 50..350 mm).
     """
     bc = bc or {"left": 2, "right": 3, "bottom": 5, "top": 5}
-    radius = radius or (lambda xx: cosine_duct_radius(xx, x_exit=x_exit))
+    base = radius or (lambda xx: cosine_duct_radius(xx, x_exit=x_exit))
+    radius = base if not bumps else (
+        lambda xx: base(xx) + hicks_henne(xx, bumps, length=x_exit))
     xs = np.array([x_exit * i / nx for i in range(nx + 1)])
     xn = np.repeat(xs[:, None], ny + 1, axis=1)
     yn = np.zeros((nx + 1, ny + 1))
@@ -220,4 +244,8 @@ WORKLOADS = {
                       lambda: bl_inlet_params(flow_channel())),
     "duct_385x129": (lambda seed=DEFAULT_SEED: duct(seed=seed), flow_duct),
     "ramp_2049x513": (lambda seed=DEFAULT_SEED: ramp(seed=seed), flow_ramp),
+    # the adjoint configuration's grid: the duct at 193x65 with 8 Hicks-Henne bumps
+    # (amplitudes 0 here; a design study sets them)
+    "duct_193x65": (lambda seed=DEFAULT_SEED: duct(nx=193, ny=65, seed=seed, bumps=[0.0] * 8),
+                    flow_duct),
 }

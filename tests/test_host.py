@@ -113,3 +113,23 @@ def test_no_gpu_fails_loudly(golden):
     with pytest.raises(tdt.TdtError) as ei:
         _create(g.mesh, g.flow)
     assert ei.value.code == lib.TDT_ERR_DEVICE
+
+
+def test_hicks_henne_bumps_and_adjoint_grid():
+    """Each Hicks-Henne bump peaks at its centre with its amplitude and vanishes
+    at both ends; the 193x65 duct has the adjoint configuration's sizes and its
+    wall moves by the bump amplitudes."""
+    from tdt import gridgen
+    for c in (50.0, 92.857, 350.0):
+        assert abs(gridgen.hicks_henne(c, [0.7], [c]) - 0.7) < 1e-12
+    assert gridgen.hicks_henne(0.0, [1.0] * 8) == 0.0
+    assert gridgen.hicks_henne(400.0, [1.0] * 8) == 0.0
+    m = gridgen.WORKLOADS["duct_193x65"][0]()
+    assert (m.neles, m.nghosts, m.nods) == (12545, 516, 12804)
+    amps = [0.3] * 8
+    flat = gridgen.duct(nx=193, ny=65, jitter=0.0)
+    bumped = gridgen.duct(nx=193, ny=65, jitter=0.0, bumps=amps)
+    top = np.arange(65, flat.nods, 66)               # the wall node of each station
+    x = flat.x_mm[top]
+    want = np.array([gridgen.hicks_henne(v, amps) for v in x])
+    assert np.allclose(bumped.y_mm[top] - flat.y_mm[top], want, atol=1e-9)
