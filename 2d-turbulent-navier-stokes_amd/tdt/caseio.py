@@ -29,11 +29,13 @@ _INT_FIELDS = {"local", "niter", "istart", "npseudo", "iaxis"}
 
 @dataclasses.dataclass
 class Flow:
-    """The 17 flow.in fields plus the three BC type-7 inputs.
+    """The 17 flow.in fields plus the inputs of BC types 7 and 6.
 
-    pinf/aminf/deltabl are never assigned by the reference (boundary.f90:312-317
-    reads them as zero); they default to 0 here and are set only by callers
-    that choose to define BC type 7.
+    pinf/aminf/deltabl (BC 7, boundary.f90:312-317) and tflame/ancomb/acomb/rhoprop
+    (BC 6: flame temperature, burn-rate exponent and coefficient, propellant
+    density; boundary.f90:138-153) are never assigned by the reference, which
+    reads them as zero; they default to 0 here and are set only by callers that
+    choose to define those boundary types.
     """
     fnamegrid: str = "grid.in"
     fnamebc: str = "bc.in"
@@ -55,6 +57,10 @@ class Flow:
     pinf: float = 0.0
     aminf: float = 0.0
     deltabl: float = 0.0
+    tflame: float = 0.0
+    ancomb: float = 0.0
+    acomb: float = 0.0
+    rhoprop: float = 0.0
 
 
 @dataclasses.dataclass

@@ -67,7 +67,8 @@ class Oracle:
             mesh.x_mm, mesh.y_mm, mesh.nod, mesh.nc, mesh.parent, mesh.ghost, mesh.btype, mesh.side)]
         dp = np.array([flow.gama, flow.amol, flow.cfl, flow.prandtl, flow.prandtlt, flow.p0,
                        flow.t0, flow.pamb, flow.amachinit, flow.rtred, flow.pinf, flow.aminf,
-                       flow.deltabl], dtype=np.float64)
+                       flow.deltabl, flow.tflame, flow.ancomb, flow.acomb, flow.rhoprop],
+                      dtype=np.float64)
         ip = np.array([flow.local, flow.npseudo, flow.iaxis], dtype=np.int32)
         k = self._keep
         self.h = L.orc_create(mesh.nods, mesh.neles, mesh.nghosts, _d(k[0]), _d(k[1]), _i(k[2]),

@@ -14,10 +14,12 @@
 !
 ! Inputs in the working directory:
 !   flow.in   (reference format, turbnstwod.f90:20-23)
-!   driver.in  NRUN IDUMP [PINF AMINF DELTABL]
+!   driver.in  NRUN IDUMP
+!              [PINF AMINF DELTABL]
+!              [TFLAME ANCOMB ACOMB RHOPROP]
 !              NRUN iterations are run; IDUMP=1 dumps iteration NRUN+1.
-!              The optional triple gives the BC type-7 inputs that the
-!              reference never assigns (boundary.f90:310-327).
+!              The optional lines give the BC type-7 and type-6 inputs that
+!              the reference never assigns (boundary.f90:138-153, 310-327).
 !===============================================================================
 PROGRAM REF_DRIVER
     USE CONSTANTS
@@ -28,8 +30,8 @@ PROGRAM REF_DRIVER
     USE BOUNDARY
     USE RESIDUAL
     IMPLICIT NONE
-    INTEGER :: NRUN, IDUMP, IOS, NMAX, IT, ISTG
-    DOUBLE PRECISION :: BLP(3), L2(5), LINF(5), STAGEA(4)
+    INTEGER :: NRUN, IDUMP, IOS, IOS6, NMAX, IT, ISTG
+    DOUBLE PRECISION :: BLP(3), B6P(4), L2(5), LINF(5), STAGEA(4)
     CHARACTER(LEN=2) :: SS
 
     OPEN(25, FILE='flow.in')
@@ -41,9 +43,14 @@ PROGRAM REF_DRIVER
     OPEN(26, FILE='driver.in')
     READ(26, *) NRUN, IDUMP
     READ(26, *, IOSTAT=IOS) BLP(1), BLP(2), BLP(3)
+    B6P = 0.0D0
+    IOS6 = 1
+    IF (IOS .EQ. 0) READ(26, *, IOSTAT=IOS6) B6P(1), B6P(2), B6P(3), B6P(4)
     CLOSE(26)
     IF (IOS .NE. 0) BLP = 0.0D0
+    IF (IOS6 .NE. 0) B6P = 0.0D0
     PINF = BLP(1); AMINF = BLP(2); DELTABL = BLP(3)
+    TFLAME = B6P(1); ANCOMB = B6P(2); ACOMB = B6P(3); RHOPROP = B6P(4)
 
     RUNIV = RUNIV_VAL
     GTH = GTH_VAL

@@ -36,6 +36,8 @@
 #define AMU_THRESH 0.001
 static const double DIV23_VAL = 2.0 / 3.0;
 static const double BL_EXPONENT = 1.0 / 7.0;
+#define CONVKSC_VAL (1.0 / 98066.5)   /* constants.f90:55 */
+#define GTH_VAL 35.0                  /* constants.f90:56 */
 
 static char g_err[256];
 const char *orc_error(void) { return g_err; }
@@ -47,7 +49,7 @@ struct orc {
     int *nc[4];
     int *parent, *ghost, *type, *side;
     double gama, amol, cfl, prandtl, prandtlt, p0, t0, pamb, amachinit, rtred, pinf, aminf,
-        deltabl;
+        deltabl, tflame, ancomb, acomb, rhoprop;
     int local, npseudo, iaxis;
     double rr, cv, cp, rho0, s0, tinf, rinit, rtol, dtmin;
     double *xcc, *ycc, *vol, *dl, *sc[8];
@@ -137,7 +139,7 @@ orc *orc_create(int32_t nods, int32_t neles, int32_t nghosts, const double *x_mm
             snprintf(g_err, sizeof g_err, "ghost id %d collides with cell ids", ghost[g]);
             return NULL;
         }
-        if (type[g] == 6 || type[g] < 2 || type[g] > 7) {
+        if (type[g] < 2 || type[g] > 7) {
             snprintf(g_err, sizeof g_err, "boundary type %d not supported", type[g]);
             return NULL;
         }
@@ -167,6 +169,7 @@ orc *orc_create(int32_t nods, int32_t neles, int32_t nghosts, const double *x_mm
     o->gama = dp[0]; o->amol = dp[1]; o->cfl = dp[2]; o->prandtl = dp[3]; o->prandtlt = dp[4];
     o->p0 = dp[5]; o->t0 = dp[6]; o->pamb = dp[7]; o->amachinit = dp[8]; o->rtred = dp[9];
     o->pinf = dp[10]; o->aminf = dp[11]; o->deltabl = dp[12];
+    o->tflame = dp[13]; o->ancomb = dp[14]; o->acomb = dp[15]; o->rhoprop = dp[16];
     o->local = ip[0]; o->npseudo = ip[1]; o->iaxis = ip[2];
 
     o->xcc = dalloc(nmax); o->ycc = dalloc(nmax); o->vol = dalloc(nmax); o->dl = dalloc(nmax);
@@ -365,7 +368,47 @@ static void ghost_fill(orc *o, int ng, double rho) {
     o->g[3][ng] = (o->c[3][ng] + p) * v;
 }
 
-/* boundary.f90:24-122, 310-327 (type 6 is rejected at creation) */
+/* DELY of cell n with the node-pair expression of face `side`
+ * (boundary.f90:176-205, 229-237) */
+static double dely_face(const orc *o, int n, int side) {
+    const double y1 = o->y[o->nod[0][n]], y2 = o->y[o->nod[1][n]];
+    const double y3 = o->y[o->nod[2][n]], y4 = o->y[o->nod[3][n]];
+    switch (side) {
+    case 1: return HALF * ((y1 + y2) - (y3 + y4));
+    case 2: return HALF * ((y2 + y3) - (y1 + y4));
+    case 3: return HALF * ((y3 + y4) - (y1 + y2));
+    default: return HALF * ((y4 + y1) - (y2 + y3));
+    }
+}
+
+static double gport_term(const orc *o, int n, double dely) {
+    return o->iaxis == 1 ? o->c[0][n] * o->u[n] * dely * o->ycc[n] : o->c[0][n] * o->u[n] * dely;
+}
+
+/* COMPUTE_GPORT + ACCUMULATE_GPORT_ALONG_BOUNDARY (boundary.f90:169-305): the
+ * mass flow through the column of cells from the parent across to the far
+ * boundary (the walk ends at the ghost whose neighbour ids point to itself,
+ * geometry.f90:145-148), normalised by the hydraulic size DHYD */
+static double gport(const orc *o, int np, int ns, double dhyd) {
+    static const int walk_nc[5] = {0, 2, 3, 0, 1};   /* NS 1->NC3, 2->NC4, 3->NC1, 4->NC2 */
+    static const int walk_side[5] = {0, 3, 4, 1, 2};
+    double g = gport_term(o, np, dely_face(o, np, ns));
+    const int side = walk_side[ns];
+    int n = o->nc[walk_nc[ns]][np];
+    g = g + gport_term(o, n, dely_face(o, n, side));
+    const int *ncw = o->nc[side - 1];
+    n = ncw[n];
+    /* SIDE 1 walks NC1 with face 3's expression, 2: NC2 / face 4, 3: NC3 / face 1,
+     * 4: NC4 / face 2 */
+    const int lside = (side + 1) % 4 + 1;
+    while (ncw[n] != n) {
+        g = g + gport_term(o, n, dely_face(o, n, lside));
+        n = ncw[n];
+    }
+    return o->iaxis == 1 ? 2.0 * g / (dhyd * dhyd) : g / dhyd;
+}
+
+/* boundary.f90:24-122, 128-164, 310-327 */
 void orc_apply_boundary(orc *o) {
     for (int gi = 1; gi <= o->nghosts; ++gi) {
         int np = o->parent[gi], ng = o->ghost[gi], nt = o->type[gi], ns = o->side[gi];
@@ -422,6 +465,36 @@ void orc_apply_boundary(orc *o) {
             o->c[4][ng] = -o->c[4][np];
             o->f[4][ng] = -o->f[4][np];
             o->g[4][ng] = -o->g[4][np];
+        } else if (nt == 6) {                            /* boundary.f90:128-164 */
+            const double dhyd = fabs(o->ycc[ng] + o->ycc[np]);
+            const double scx = o->sc[2 * (ns - 1)][np], scy = o->sc[2 * (ns - 1) + 1][np];
+            const double dnr = sqrt(scx * scx + scy * scy);
+            const double enx = -scx / dnr, eny = -scy / dnr;
+            const double gp = gport(o, np, ns, dhyd);
+            double rho = o->c[0][np];
+            const double rdot = o->acomb * pow(P[np] * CONVKSC_VAL, o->ancomb);
+            double rdoteb = rdot;
+            double vel = o->rhoprop / rho * rdoteb;
+            P[ng] = P[np];
+            U[ng] = vel * enx + vel * enx - U[np];
+            V[ng] = vel * eny + vel * eny - V[np];
+            T[ng] = o->tflame - 0.5 * (U[ng] * U[ng] + V[ng] * V[ng]) / o->cp;
+            double amu = (0.00000011848) * sqrt(o->amol) * pow(T[ng], 0.6);
+            const double gval = (gp / (o->rhoprop * rdot)) *
+                                pow(o->rhoprop * rdot * dhyd / amu, -0.125);
+            if (gval > GTH_VAL)
+                rdoteb = rdoteb * (1.0 + 0.023 * (pow(gval, 0.8) - pow(GTH_VAL, 0.8)));
+            vel = o->rhoprop / rho * rdoteb;
+            U[ng] = vel * enx + vel * enx - U[np];
+            V[ng] = vel * eny + vel * eny - V[np];
+            T[ng] = o->tflame - 0.5 * (U[ng] * U[ng] + V[ng] * V[ng]) / o->cp;
+            rho = P[ng] / (o->rr * T[ng]);
+            amu = (0.00000011848) * sqrt(o->amol) * pow(T[ng], 0.6);
+            const double rt = amu / rho;
+            ghost_fill(o, ng, rho);
+            o->c[4][ng] = rho * rt;
+            o->f[4][ng] = o->c[4][ng] * U[ng];
+            o->g[4][ng] = o->c[4][ng] * V[ng];
         } else if (nt == 7) {                            /* boundary.f90:310-327 */
             P[ng] = o->pinf; T[ng] = o->tinf; V[ng] = -V[np];
             double rho = P[ng] / (o->rr * T[ng]);

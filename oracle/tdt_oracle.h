@@ -20,6 +20,7 @@ extern "C" {
 typedef struct orc orc;
 
 /* dparams: gama amol cfl prandtl prandtlt p0 t0 pamb amachinit rtred pinf aminf deltabl
+ *          tflame ancomb acomb rhoprop
  * iparams: local npseudo iaxis
  * nod, nc: column-major [4][neles], 1-based ids (NOD(I,K) = nod[(K-1)*neles + I-1]). */
 orc *orc_create(int32_t nods, int32_t neles, int32_t nghosts, const double *x_mm,

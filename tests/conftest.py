@@ -40,7 +40,8 @@ class Golden:
         return self.z["d_" + tag]
 
 
-DUMP_CASES = ["t2d_chan_cold", "t2d_chan", "t2d_ramp", "taxi_duct", "t7_plate"]
+DUMP_CASES = ["t2d_chan_cold", "t2d_chan", "t2d_ramp", "taxi_duct", "t7_plate", "t6_port",
+              "t6_port_axi"]
 RUN_CASES = ["case1", "case1_axi", "duct_385x129"]
 
 

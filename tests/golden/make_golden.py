@@ -62,7 +62,8 @@ def run_case(name, mesh, flow, nrun, dump, sample_stride=1, crosscheck=True):
         flow = dataclasses.replace(flow, niter=nrun + (1 if dump else 0), istart=0)
         caseio.write_case(work, flow, mesh)
         with open(os.path.join(work, "driver.in"), "w") as fh:
-            fh.write(f"{nrun} {1 if dump else 0}\n{flow.pinf!r} {flow.aminf!r} {flow.deltabl!r}\n")
+            fh.write(f"{nrun} {1 if dump else 0}\n{flow.pinf!r} {flow.aminf!r} {flow.deltabl!r}\n"
+                     f"{flow.tflame!r} {flow.ancomb!r} {flow.acomb!r} {flow.rhoprop!r}\n")
         subprocess.check_call([os.path.join(REF, "ref_driver")], cwd=work, stdout=subprocess.DEVNULL)
         final = parse_final(os.path.join(work, "final.bin"))
         l2, linf = parse_hist(os.path.join(work, "hist.bin"))
@@ -142,9 +143,22 @@ def tiny_duct_radius(x):
                                       x_exit=40.0)
 
 
+def port_flow(iaxis, mach):
+    """A propellant port at flame temperature: BC type 6 (mass injection with the
+    reference's burn-rate law and erosive correction), its inputs defined here."""
+    return gridgen.flow_channel(p0=7.0e6, t0=2500.0, amachinit=mach, iaxis=iaxis, tflame=2500.0,
+                                ancomb=0.35, acomb=1.2e-3, rhoprop=1700.0)
+
+
 def main():
     if not os.path.exists(os.path.join(REF, "ref_driver")):
         raise SystemExit("oracle/_ref not built: run oracle/build_ref.sh first")
+    only = set(sys.argv[1:])
+    if only:                                      # regenerate just the named fixtures
+        global run_case, run_host_case
+        _rc, _rh = run_case, run_host_case
+        run_case = lambda name, *a, **k: _rc(name, *a, **k) if name in only else None  # noqa: E731
+        run_host_case = lambda name, *a, **k: _rh(name, *a, **k) if name in only else None  # noqa: E731
     # 2-D channel: inlet 2, outlet 3 (extrapolated pressure), wall 5, symmetry 4
     m = gridgen.channel(nx=10, ny=6, length=50.0, height=10.0, h1=0.5, jitter=0.03,
                         bc={"left": 2, "right": 3, "bottom": 5, "top": 4}, seed=11)
@@ -167,6 +181,16 @@ def main():
                         bc={"left": 7, "right": 3, "bottom": 5, "top": 4}, seed=14)
     f = gridgen.bl_inlet_params(gridgen.flow_channel(amachinit=0.3), deltabl_m=0.004)
     run_case("t7_plate", m, f, nrun=20, dump=True, crosscheck=False)
+    # BC type 6 (burning propellant surface, mass injection) with its inputs defined;
+    # 2-D port: burning floor, symmetry top, closed head end, extrapolated exit
+    m = gridgen.channel(nx=12, ny=6, length=60.0, height=10.0, h1=0.3, jitter=0.02,
+                        bc={"left": 5, "right": 3, "bottom": 6, "top": 4}, seed=15)
+    run_case("t6_port", m, port_flow(0, 0.02), nrun=20, dump=True, crosscheck=False)
+    # axisymmetric port: axis at y = 0 (symmetry), burning outer wall; the fast core
+    # flow takes the erosive-burning branch (GVAL > GTH)
+    m = gridgen.channel(nx=12, ny=6, length=60.0, height=10.0, h1=0.3, jitter=0.02,
+                        bc={"left": 5, "right": 3, "bottom": 4, "top": 6}, seed=16)
+    run_case("t6_port_axi", m, port_flow(1, 0.8), nrun=20, dump=True, crosscheck=False)
     # the reference's example case (examples/case1), 2-D and axisymmetric
     c1 = os.path.join(REFERENCE, "examples", "case1")
     if os.path.isdir(c1):
