@@ -65,6 +65,9 @@ Consts<R> make_consts(const tdt_params &P) {
     K.sqcmu = (R)std::sqrt(2.0 * std::sqrt(tdt::kCmu));
     K.bl_u = (R)(2.0 * P.aminf * std::sqrt(P.gama * rr * tinf));
     K.gamma_rr = (R)(P.gama * rr);
+    K.tflame = (R)P.tflame; K.ancomb = (R)P.ancomb; K.acomb = (R)P.acomb;
+    K.rhoprop = (R)P.rhoprop;
+    K.gth_08 = (R)std::pow(tdt::kGth, 0.8);
     K.local = P.local; K.npseudo = P.npseudo; K.iaxis = P.iaxis;
     K.pamb_set = P.pamb >= 0.0;
     return K;
@@ -213,7 +216,10 @@ int tdt_create(tdt_ctx **out, const tdt_mesh *mesh, const tdt_params *P, int32_t
         *gopp = dev_alloc<int>(c, g);
     int *nptr = dev_alloc<int>(c, m + 1), *nlist = dev_alloc<int>(c, pm.nlist.size());
     int4 *novr = dev_alloc<int4>(c, m);
-    ok = nc4 && nod4 && xn && yn && gpar && gside && gty && gopp && nptr && nlist && novr;
+    int *g6ptr = dev_alloc<int>(c, g + 1), *g6cell = dev_alloc<int>(c, pm.g6cell.size());
+    double *g6dely = dev_alloc<double>(c, pm.g6dely.size());
+    ok = nc4 && nod4 && xn && yn && gpar && gside && gty && gopp && nptr && nlist && novr &&
+         g6ptr && g6cell && g6dely;
     up(nc4, pm.nc4.data(), n * sizeof(int4));
     up(nod4, pm.nod4.data(), n * sizeof(int4));
     up(xn, pm.xn.data(), m * sizeof(double));
@@ -225,9 +231,13 @@ int tdt_create(tdt_ctx **out, const tdt_mesh *mesh, const tdt_params *P, int32_t
     up(nptr, pm.nptr.data(), (m + 1) * sizeof(int));
     up(nlist, pm.nlist.data(), pm.nlist.size() * sizeof(int));
     up(novr, pm.novr.data(), m * sizeof(int4));
+    up(g6ptr, pm.g6ptr.data(), (g + 1) * sizeof(int));
+    up(g6cell, pm.g6cell.data(), pm.g6cell.size() * sizeof(int));
+    up(g6dely, pm.g6dely.data(), pm.g6dely.size() * sizeof(double));
     D.nc4 = nc4; D.nod4 = nod4; D.xn = xn; D.yn = yn;
     D.gparent = gpar; D.gside = gside; D.gtype = gty; D.gopp = gopp;
     D.nptr = nptr; D.nlist = nlist; D.novr = novr;
+    D.g6ptr = g6ptr; D.g6cell = g6cell; D.g6dely = g6dely;
     if (ok) {
         D.xcc = dev_alloc<double>(c, nt); D.ycc = dev_alloc<double>(c, nt);
         D.vol = dev_alloc<double>(c, nt); D.dl = dev_alloc<double>(c, n);

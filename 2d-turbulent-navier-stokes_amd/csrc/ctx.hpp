@@ -28,6 +28,8 @@ struct Dev {
     const int *gparent, *gside, *gtype, *gopp;  // [g]
     const int *nptr, *nlist;  // node gather lists
     const int4 *novr;         // [m]
+    const int *g6ptr, *g6cell;  // BC type 6 port columns (mesh_prep.hpp)
+    const double *g6dely;
     double *xcc, *ycc, *vol;  // [nt]
     double *dl;               // [n]
     double *sc;               // [8][n] SC1X SC1Y SC2X SC2Y SC3X SC3Y SC4X SC4Y

@@ -20,13 +20,18 @@
 // Every kernel is a cell- or node-centric gather: no atomics on values, so
 // results are deterministic.  Ghost neighbours of a cell are only ever
 // consumed by that cell (its parent), which is what lets the stage kernels
-// compute them on the fly instead of running a separate boundary pass.
+// compute them on the fly instead of running a separate boundary pass.  The
+// exception is BC type 6, whose ghost also depends on the mass flow of a whole
+// column of cells: on grids that have such ghosts, k_bc6 forms them once per
+// stage ahead of the kernels that read them (the kernels are then instantiated
+// with that load path; grids without type 6 run kernels compiled without it).
 #include "fused_kernels.hpp"
 
 namespace tdt {
 
 hipError_t enqueue_fp64(tdt_ctx *c, FusedBufs &B) {
-    return enqueue_iteration<double, double, double>(c, B, c->K);
+    return B.F.has_bc6 ? enqueue_iteration<double, double, double, true>(c, B, c->K)
+                       : enqueue_iteration<double, double, double, false>(c, B, c->K);
 }
 
 namespace {
@@ -74,7 +79,9 @@ int fused_setup(tdt_ctx *c) {
     F.n = D.n; F.g = D.g; F.m = D.m; F.nt = D.nt;
     F.remap = 1;
     F.nc4 = D.nc4; F.nod4 = D.nod4; F.xn = D.xn; F.yn = D.yn;
-    F.gparent = D.gparent; F.gtype = D.gtype; F.gopp = D.gopp;
+    F.gparent = D.gparent; F.gtype = D.gtype; F.gopp = D.gopp; F.gside = D.gside;
+    F.g6ptr = D.g6ptr; F.g6cell = D.g6cell; F.g6dely = D.g6dely;
+    F.has_bc6 = !c->pm.g6cell.empty();
     F.nptr = D.nptr; F.nlist = D.nlist; F.novr = D.novr;
     F.xcc = D.xcc; F.ycc = D.ycc; F.vol = D.vol; F.dl = D.dl; F.sc = D.sc;
     F.gxcc = D.xcc; F.gycc = D.ycc; F.gvol = D.vol; F.gsc = D.sc; F.gxn = D.xn; F.gyn = D.yn;
@@ -105,6 +112,10 @@ int fused_setup(tdt_ctx *c) {
         F.gxn = dalloc<float>(*B, D.m);
         F.gyn = dalloc<float>(*B, D.m);
         ok = ok && F.gxcc && F.gycc && F.gvol && F.gsc && F.gxn && F.gyn;
+    }
+    if (F.has_bc6) {
+        F.g6s = dalloc<double>(*B, (size_t)6 * D.g);
+        ok = ok && F.g6s;
     }
     B->nb_cells = (int)nblocks(D.n);
     F.ncb8 = (B->nb_cells + 7) & ~7;
@@ -278,7 +289,7 @@ static hipError_t autotune(tdt_ctx *c, FusedBufs &B) {
     }
     // per slot: pre-pass (merged's k_pre vs quad's kq_pre), stage 1 (merged,
     // split = k_dv + stage 1, quad), stages 2-4 (merged's vs quad's)
-    int comp = (t[2][0] < t[0][0]) ? kPreQuad : 0;
+    int comp = t[2][0] < t[0][0] ? kPreQuad : 0;
     const double s1[3] = {t[0][2], t[1][1] + t[1][2], t[2][2]};
     const int s1set[3] = {0, kSt1Split, kSt1Quad};
     int b1 = 0;

@@ -35,7 +35,11 @@ struct FDev {
     const int *nrel;                // [m] count | U, V, nu~ sum starts << 4, 8, 12 |
                                     //   nu~ floor flag << 16
     const double *xn, *yn;
-    const int *gparent, *gtype, *gopp;
+    const int *gparent, *gtype, *gopp, *gside;
+    const int *g6ptr, *g6cell;      // BC type 6 port columns
+    const double *g6dely;
+    int has_bc6;                    // any type-6 ghost
+    void *g6s;                      // [6][g] type-6 ghosts of the current stage (k_bc6)
     const int *nptr, *nlist;
     const int4 *novr;
     const double *xcc, *ycc, *vol, *dl, *sc;

@@ -58,9 +58,86 @@ __device__ __forceinline__ void fold_monitor(const FDev &F, int nblk) {
     }
 }
 
+// BC type 6 ghost k (parent par) from the cells' states in q (storage type T,
+// component stride qs): the port column's C1, C2, the parent face's surface
+// vector and the centroids, all as the flux arithmetic reads them.
+template <class R, class T, class SD = R>
+__device__ __forceinline__ void bc6_fused(const FDev &F, Prim<R> &s, const Prim<R> &pp, int k,
+                                          int par, const void *q, size_t qs,
+                                          const Consts<R> &KR) {
+    const int f = F.gside[k] - 1, n = F.n;
+    const int gid = n + k;
+    bc6_state<R>(s, pp, (R)ld<SD>(F.gsc, (size_t)(2 * f) * n + par),
+                 (R)ld<SD>(F.gsc, (size_t)(2 * f + 1) * n + par),
+                 (R)fabs(ld<SD>(F.gycc, gid) + ld<SD>(F.gycc, par)), F.g6ptr[k], F.g6ptr[k + 1],
+                 F.g6cell, F.g6dely,
+                 [&](int c, R &c1, R &c2, R &yc) {
+                     c1 = (R)ld<T>(q, c);
+                     c2 = (R)ld<T>(q, qs + c);
+                     yc = (R)ld<SD>(F.gycc, c);
+                 },
+                 KR);
+}
+
+// k_bc6: the type-6 ghosts of one stage's input state, each from its parent and
+// its port column (a walk across the grid), formed once per stage so that the
+// cell kernels only load them: F.g6s[6][g] = U, V, T, P, rho, C5 in the
+// arithmetic type R (other ghosts' entries are not written).
+template <class R, class SI, class SD, int STAGE>
+__global__ void __launch_bounds__(kBlock) k_bc6(FDev F, Consts<R> KR) {
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= F.g || F.gtype[k] != 6) return;
+    const void *qin = STAGE == 1 ? (const void *)F.qn : (const void *)F.st[STAGE & 1];
+    const size_t sin = STAGE == 1 ? (size_t)F.nt : (size_t)F.n;
+    const int par = F.gparent[k], g = F.g;
+    Prim<R> pp, s;
+    prim_from_q<R>(pp, (R)ld<SI>(qin, par), (R)ld<SI>(qin, sin + par),
+                   (R)ld<SI>(qin, 2 * sin + par), (R)ld<SI>(qin, 3 * sin + par),
+                   (R)ld<SI>(qin, 4 * sin + par), KR);
+    bc6_fused<R, SI, SD>(F, s, pp, k, par, qin, sin, KR);
+    stv<R>(F.g6s, k, s.u);
+    stv<R>(F.g6s, (size_t)g + k, s.v);
+    stv<R>(F.g6s, (size_t)2 * g + k, s.t);
+    stv<R>(F.g6s, (size_t)3 * g + k, s.p);
+    stv<R>(F.g6s, (size_t)4 * g + k, s.c[0]);
+    stv<R>(F.g6s, (size_t)5 * g + k, s.c[4]);
+}
+
+// The type-6 ghost k as k_bc6 left it: the values a node gather reads ...
+template <class R>
+__device__ __forceinline__ void bc6_load(const FDev &F, int k, R &u, R &v, R &t, R &rho, R &c5) {
+    const size_t g = F.g;
+    u = (R)ld<R>(F.g6s, k);
+    v = (R)ld<R>(F.g6s, g + k);
+    t = (R)ld<R>(F.g6s, 2 * g + k);
+    rho = (R)ld<R>(F.g6s, 4 * g + k);
+    c5 = (R)ld<R>(F.g6s, 5 * g + k);
+}
+
+// ... and all six, for ghost_state.
+template <class R>
+struct Bc6Load {
+    const FDev &F;
+    int k;
+    __device__ void operator()(R &u, R &v, R &t, R &p, R &rho, R &c5) const {
+        bc6_load<R>(F, k, u, v, t, rho, c5);
+        p = (R)ld<R>(F.g6s, (size_t)3 * F.g + k);
+    }
+};
+
+// The loader ghost_state takes for ghost k: the type-6 states when the grid has
+// them (B6); otherwise none, and the kernels compile as for a grid without.
+template <bool B6, class R>
+__device__ __forceinline__ auto g6_for(const FDev &F, int k) {
+    if constexpr (B6)
+        return Bc6Load<R>{F, k};
+    else
+        return NoBc6{};
+}
+
 // Node gather through the contributor list, for nodes with more than
 // kNodeWidth contributors (viscous.f90:77-124 in the reference's order).
-template <class R>
+template <class R, bool B6>
 __device__ __forceinline__ void node_gather_list(const FDev &F, const Consts<R> &KR, int v, R &au,
                                                  R &av, R &at, R &arho, R &ar) {
     const int nt = F.nt;
@@ -85,8 +162,12 @@ __device__ __forceinline__ void node_gather_list(const FDev &F, const Consts<R> 
             if (ty == 3)
                 uvtp_from_q<R>((R)Q[opp], (R)Q[nt + opp], (R)Q[2 * nt + opp], (R)Q[3 * nt + opp],
                                uo, vo, to, po, KR);
-            ghost_state<R>(s, ty, pp, uo, vo, to, po, (R)F.ycc[id], KR);
-            u = s.u; vv = s.v; t = s.t; rho = s.c[0]; c5 = s.c[4];
+            if (B6 && ty == 6) {
+                bc6_load<R>(F, k, u, vv, t, rho, c5);
+            } else {
+                ghost_state<R>(s, ty, pp, uo, vo, to, po, (R)F.ycc[id], KR);
+                u = s.u; vv = s.v; t = s.t; rho = s.c[0]; c5 = s.c[4];
+            }
         }
         R rr = fdiv(c5, rho);
         if (j >= ov.x) au += u;
@@ -102,7 +183,7 @@ __device__ __forceinline__ void node_gather_list(const FDev &F, const Consts<R> 
 // to multiples of 8 and spread so that each XCD works on contiguous cells and
 // the nodes around them), and a last block of its own that folds the previous
 // iteration's monitor partials (off every cell's path).
-template <class SD, class R>
+template <class SD, class R, bool B6>
 __global__ void __launch_bounds__(kBlock) k_pre(FDev F, Consts<double> K, Consts<R> KR,
                                                 int nb_partials, int ncb8, int nnb8) {
     int b = blockIdx.x;
@@ -196,8 +277,12 @@ __global__ void __launch_bounds__(kBlock) k_pre(FDev F, Consts<double> K, Consts
                                    vo, to, po, KR);
                 else
                     ycg = (R)ld<SD>(F.gycc, a);
-                ghost_state<R>(s, ty, pp, uo, vo, to, po, ycg, KR);
-                u = s.u; vv = s.v; t = s.t; rho = s.c[0]; c5 = s.c[4];
+                if (B6 && ty == 6) {
+                    bc6_load<R>(F, a - n, u, vv, t, rho, c5);
+                } else {
+                    ghost_state<R>(s, ty, pp, uo, vo, to, po, ycg, KR);
+                    u = s.u; vv = s.v; t = s.t; rho = s.c[0]; c5 = s.c[4];
+                }
             }
             const R rr = fdiv(c5, rho);
             if (k >= su) au += u;
@@ -208,7 +293,7 @@ __global__ void __launch_bounds__(kBlock) k_pre(FDev F, Consts<double> K, Consts
         }
     } else {
         ar = F.novr[v].w ? KR.rtol_rtred : R(0);
-        node_gather_list<R>(F, KR, v, au, av, at, arho, ar);
+        node_gather_list<R, B6>(F, KR, v, au, av, at, arho, ar);
     }
     const int m = F.m;
     const R nc = (R)(cnt <= kNodeWidth ? cnt : F.nptr[v + 1] - F.nptr[v]);
@@ -223,7 +308,7 @@ __global__ void __launch_bounds__(kBlock) k_pre(FDev F, Consts<double> K, Consts
 // ---------------------------------------------------------------------------------
 // k_dv: dissipation D and viscous fluxes VIS from Q^n in a kernel of their own
 // (the split variant; the merged variant forms them inside stage 1).
-template <class SD, class R>
+template <class SD, class R, bool B6>
 __global__ void __launch_bounds__(kBlock) k_dv(FDev F, Consts<double> K, Consts<R> KR) {
     int i = cell_block(F) * kBlock + threadIdx.x;
     if (i >= F.n) return;
@@ -294,8 +379,12 @@ __global__ void __launch_bounds__(kBlock) k_dv(FDev F, Consts<double> K, Consts<
                 uvtp_from_q<R>((R)Q[o], (R)Q[nt + o], (R)Q[2 * nt + o], (R)Q[3 * nt + o], uo, vo,
                                to, po, KR);
             Prim<R> s;
-            ghost_state<R>(s, ty, pi, uo, vo, to, po, (R)ld<SD>(F.gycc, c), KR);
-            uc = s.u; vc = s.v; tc = s.t; rhoc = s.c[0]; c5c = s.c[4];
+            if (B6 && ty == 6) {
+                bc6_load<R>(F, c - n, uc, vc, tc, rhoc, c5c);
+            } else {
+                ghost_state<R>(s, ty, pi, uo, vo, to, po, (R)ld<SD>(F.gycc, c), KR);
+                uc = s.u; vc = s.v; tc = s.t; rhoc = s.c[0]; c5c = s.c[4];
+            }
         }
         R rc = fdiv(c5c, rhoc);
         ViscFace<R> o = viscous_face<R>(
@@ -328,7 +417,7 @@ __global__ void __launch_bounds__(kBlock) k_dv(FDev F, Consts<double> K, Consts<
 // its share of the source-term gradient differences.  SI / SO: stage input /
 // output storage types; SD: storage of D, VIS, sensors and nodal averages.
 // DV = false: stage 1 reads D and VIS written by k_dv (the split variant).
-template <class R, class SI, class SO, class SD, int STAGE, bool DV = true>
+template <class R, class SI, class SO, class SD, int STAGE, bool DV, bool B6>
 __global__ void __launch_bounds__(kBlock) k_rk(FDev F, Consts<double> K, Consts<R> KR,
                                                double alpha) {
     constexpr bool kInlineDV = STAGE == 1 && DV;
@@ -443,7 +532,8 @@ __global__ void __launch_bounds__(kBlock) k_rk(FDev F, Consts<double> K, Consts<
                                        uo, vo, to, po, KR);
                     }
                 }
-                ghost_state<R>(pj, ty, pi, uo, vo, to, po, (R)ld<SD>(F.gycc, jn), KR);
+                ghost_state<R>(pj, ty, pi, uo, vo, to, po, (R)ld<SD>(F.gycc, jn), KR,
+                               g6_for<B6, R>(F, jn - n));
             }
             // central flux terms of this face (residual.f90:34-54)
             const R sx = (R)ld<SD>(F.gsc, (2 * f) * n + i);
@@ -560,7 +650,7 @@ void launch(tdt_ctx *c, FusedBufs &B, int slot, K kern, unsigned grid, Args... a
         hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, c->stream, args...);
 }
 
-template <class R, class SS, class SD>
+template <class R, class SS, class SD, bool B6>
 hipError_t enqueue_iteration(tdt_ctx *c, FusedBufs &B, const Consts<R> &KR) {
     const FDev &F = B.F;
     const int ks = B.kset;
@@ -572,33 +662,46 @@ hipError_t enqueue_iteration(tdt_ctx *c, FusedBufs &B, const Consts<R> &KR) {
             (void)hipEventRecord(B.ev[3], c->stream);
         }
     };
+    // type-6 ghosts of each stage's input, ahead of the kernels that read them
+    const unsigned ngb = (unsigned)((F.g + kBlock - 1) / kBlock);
+    auto bc6 = [&](auto kern) {
+        if (B6) hipLaunchKernelGGL(kern, dim3(ngb), dim3(kBlock), 0, c->stream, F, KR);
+    };
+    bc6(k_bc6<R, double, SD, 1>);
     if (ks & kPreQuad) {
-        launch(c, B, 0, kq_pre<SD, R>, nq + B.nbq_nodes + 1, F, c->K, KR, B.nbq_cells, nparts);
+        launch(c, B, 0, kq_pre<SD, R, B6>, nq + B.nbq_nodes + 1, F, c->K, KR, B.nbq_cells,
+               nparts);
     } else {
         const int nnb8 = (B.nb_nodes + 7) & ~7;
-        launch(c, B, 0, k_pre<SD, R>, F.ncb8 + nnb8 + 1, F, c->K, KR, nparts, F.ncb8, nnb8);
+        launch(c, B, 0, k_pre<SD, R, B6>, F.ncb8 + nnb8 + 1, F, c->K, KR, nparts, F.ncb8, nnb8);
     }
     switch (ks & kSt1Mask) {
     case kSt1Split:
-        launch(c, B, 1, k_dv<SD, R>, nbc, F, c->K, KR);
-        launch(c, B, 2, k_rk<R, double, SS, SD, 1, false>, nbc, F, c->K, KR, 0.25);
+        launch(c, B, 1, k_dv<SD, R, B6>, nbc, F, c->K, KR);
+        launch(c, B, 2, k_rk<R, double, SS, SD, 1, false, B6>, nbc, F, c->K, KR, 0.25);
         break;
     case kSt1Quad:
         no_dv_slot();
-        launch(c, B, 2, kq_stage<R, double, SS, SD, 1>, nq, F, c->K, KR, 0.25);
+        launch(c, B, 2, kq_stage<R, double, SS, SD, 1, B6>, nq, F, c->K, KR, 0.25);
         break;
     default:
         no_dv_slot();
-        launch(c, B, 2, k_rk<R, double, SS, SD, 1>, nbc, F, c->K, KR, 0.25);
+        launch(c, B, 2, k_rk<R, double, SS, SD, 1, true, B6>, nbc, F, c->K, KR, 0.25);
     }
     if (ks & kStQuad) {
-        launch(c, B, 3, kq_stage<R, SS, SS, SD, 2>, nq, F, c->K, KR, 1.0 / 3.0);
-        launch(c, B, 4, kq_stage<R, SS, SS, SD, 3>, nq, F, c->K, KR, 0.5);
-        launch(c, B, 5, kq_stage<R, SS, double, SD, 4>, nq, F, c->K, KR, 1.0);
+        bc6(k_bc6<R, SS, SD, 2>);
+        launch(c, B, 3, kq_stage<R, SS, SS, SD, 2, B6>, nq, F, c->K, KR, 1.0 / 3.0);
+        bc6(k_bc6<R, SS, SD, 3>);
+        launch(c, B, 4, kq_stage<R, SS, SS, SD, 3, B6>, nq, F, c->K, KR, 0.5);
+        bc6(k_bc6<R, SS, SD, 4>);
+        launch(c, B, 5, kq_stage<R, SS, double, SD, 4, B6>, nq, F, c->K, KR, 1.0);
     } else {
-        launch(c, B, 3, k_rk<R, SS, SS, SD, 2>, nbc, F, c->K, KR, 1.0 / 3.0);
-        launch(c, B, 4, k_rk<R, SS, SS, SD, 3>, nbc, F, c->K, KR, 0.5);
-        launch(c, B, 5, k_rk<R, SS, double, SD, 4>, nbc, F, c->K, KR, 1.0);
+        bc6(k_bc6<R, SS, SD, 2>);
+        launch(c, B, 3, k_rk<R, SS, SS, SD, 2, true, B6>, nbc, F, c->K, KR, 1.0 / 3.0);
+        bc6(k_bc6<R, SS, SD, 3>);
+        launch(c, B, 4, k_rk<R, SS, SS, SD, 3, true, B6>, nbc, F, c->K, KR, 0.5);
+        bc6(k_bc6<R, SS, SD, 4>);
+        launch(c, B, 5, k_rk<R, SS, double, SD, 4, true, B6>, nbc, F, c->K, KR, 1.0);
     }
     return hipGetLastError();
 }

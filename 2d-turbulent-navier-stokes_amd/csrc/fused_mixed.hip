@@ -8,7 +8,8 @@
 namespace tdt {
 
 hipError_t enqueue_mixed(tdt_ctx *c, FusedBufs &B) {
-    return enqueue_iteration<float, float, float>(c, B, c->Kf);
+    return B.F.has_bc6 ? enqueue_iteration<float, float, float, true>(c, B, c->Kf)
+                       : enqueue_iteration<float, float, float, false>(c, B, c->Kf);
 }
 
 }  // namespace tdt

@@ -110,10 +110,6 @@ std::string prepare_mesh(const tdt_mesh &ms, int ordering, PreparedMesh &pm, int
         if (gid <= n) return fmt("bc.in record %ld: ghost id %ld collides with a cell id", k + 1, gid);
         if (par < 1 || par > n) return fmt("bc.in record %ld: parent %ld out of range", k + 1, par);
         if (sd < 1 || sd > 4) return fmt("bc.in record %ld: side %ld out of range", k + 1, sd);
-        if (ty == 6) {
-            *code = TDT_ERR_UNSUPPORTED;
-            return fmt("bc.in record %ld: boundary type 6 (combustion) is not supported", k + 1);
-        }
         if (ty < 2 || ty > 7) return fmt("bc.in record %ld: unknown boundary type %ld", k + 1, ty);
         if (!ghost_index.emplace(gid, k).second)
             return fmt("bc.in record %ld: ghost id %ld listed twice", k + 1, gid);
@@ -253,11 +249,53 @@ std::string prepare_mesh(const tdt_mesh &ms, int ordering, PreparedMesh &pm, int
         // opposite face: side 1 -> NC3, 2 -> NC4, 3 -> NC1, 4 -> NC2 (boundary.f90:78-82)
         int opp = pm.nc4[(size_t)4 * par + ((ms.side[k] + 1) & 3)];
         pm.gopp[k] = opp;
-        if (ms.type[k] == 3 && opp >= n) {
+        if ((ms.type[k] == 3 || ms.type[k] == 6) && opp >= n) {
             *code = TDT_ERR_UNSUPPORTED;
-            return fmt("bc.in record %ld: type-3 ghost whose opposite neighbour is also a ghost",
-                       k + 1);
+            return fmt("bc.in record %ld: type-%ld ghost whose opposite neighbour is also a ghost",
+                       k + 1, ms.type[k]);
         }
+    }
+    // ---- BC type 6 (boundary.f90:169-305): the column of cells from the parent
+    // across to the far boundary (the reference's walk ends at the first ghost),
+    // each with the DELY expression the reference uses for it: the parent with
+    // its own side's, the first cell with the walking side's, the rest with the
+    // opposite one's
+    auto dely = [&](int c, int side) {
+        const double y1 = pm.yn[pm.nod4[(size_t)4 * c]], y2 = pm.yn[pm.nod4[(size_t)4 * c + 1]];
+        const double y3 = pm.yn[pm.nod4[(size_t)4 * c + 2]], y4 = pm.yn[pm.nod4[(size_t)4 * c + 3]];
+        switch (side) {
+        case 1: return 0.5 * ((y1 + y2) - (y3 + y4));
+        case 2: return 0.5 * ((y2 + y3) - (y1 + y4));
+        case 3: return 0.5 * ((y3 + y4) - (y1 + y2));
+        default: return 0.5 * ((y4 + y1) - (y2 + y3));
+        }
+    };
+    pm.g6ptr.assign(g + 1, 0);
+    pm.g6cell.clear();
+    pm.g6dely.clear();
+    for (int k = 0; k < g; ++k) {
+        if (ms.type[k] == 6) {
+            const int par = pm.gparent[k], ns = ms.side[k];
+            const int side = (ns + 1) % 4 + 1;          // 1 -> 3, 2 -> 4, 3 -> 1, 4 -> 2
+            const int lside = (side + 1) % 4 + 1;
+            pm.g6cell.push_back(par);
+            pm.g6dely.push_back(dely(par, ns));
+            int cur = pm.gopp[k];                       // NC(side)(parent), a cell
+            pm.g6cell.push_back(cur);
+            pm.g6dely.push_back(dely(cur, side));
+            cur = pm.nc4[(size_t)4 * cur + side - 1];
+            for (int steps = 0; cur < n; ++steps) {
+                if (steps >= n) {
+                    *code = TDT_ERR_UNSUPPORTED;
+                    return fmt("bc.in record %ld: the type-6 column walk reaches no boundary",
+                               k + 1);
+                }
+                pm.g6cell.push_back(cur);
+                pm.g6dely.push_back(dely(cur, lside));
+                cur = pm.nc4[(size_t)4 * cur + side - 1];
+            }
+        }
+        pm.g6ptr[k + 1] = (int)pm.g6cell.size();
     }
     // ---- node gather lists in the reference accumulation order ----
     std::vector<std::vector<int>> lists(m);

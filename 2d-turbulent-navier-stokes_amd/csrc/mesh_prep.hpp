@@ -37,6 +37,12 @@ struct PreparedMesh {
     // when the nu~ sum restarts from RTOL*RTRED (type 5) instead of 0.
     std::vector<int> nptr, nlist;
     std::vector<int> novr;           // [m][4]
+
+    // BC type 6: for ghost k, entries g6ptr[k] .. g6ptr[k+1] list the cells whose
+    // mass flow C1*U*DELY (*YCC) sums to the port flow, in the reference's order,
+    // with their DELY (empty for other types)
+    std::vector<int> g6ptr, g6cell;
+    std::vector<double> g6dely;
 };
 
 // Returns "" on success, otherwise an error message; *code receives the

@@ -6,6 +6,8 @@
 // library.  R = float is the FP32-mixed arithmetic.
 #pragma once
 
+#include <type_traits>
+
 #include <hip/hip_runtime.h>
 
 #include "tmath.hpp"
@@ -22,6 +24,8 @@ constexpr double kDiv23 = 2.0 / 3.0;
 constexpr double kYaxisThresh = 0.001, kAmuThresh = 0.001;
 constexpr double kBlExponent = 1.0 / 7.0;
 constexpr double kPaToBar = 0.00001;
+constexpr double kConvKsc = 1.0 / 98066.5;   // Pa -> kgf/cm^2 (constants.f90:55)
+constexpr double kGth = 35.0;                // erosive-burning threshold (constants.f90:56)
 
 // Flow constants derived once on the host with the reference's expressions.
 template <class R>
@@ -34,6 +38,8 @@ struct Consts {
     R sqcmu;                   // DSQRT(2*DSQRT(CMU))
     R bl_u;                    // 2*AMINF*DSQRT(GAMA*RR*TINF)
     R gamma_rr;                // GAMA*RR
+    R tflame, ancomb, acomb, rhoprop;  // BC type 6 inputs
+    R gth_08;                  // GTH**0.8 (constants.f90:56), from the host's libm
     int local, npseudo, iaxis, pamb_set;
 };
 
@@ -171,12 +177,34 @@ __device__ __forceinline__ void ghost_fill(Prim<R> &s, R rho, const Consts<R> &K
     s.g[3] = (s.c[3] + s.p) * s.v;
 }
 
-// Ghost-cell state for boundary types 2, 3, 4, 5, 7 (boundary.f90:53-122, 310-327).
-// par: parent state; (uo, vo, to, po): primitives of the parent's opposite
-// neighbour (type 3 only); ycc_g: ghost centroid y (type 7 only).
+// Common tail of the ghosts that get a new state (types 2, 6, 7): C, F, G from
+// the primitives in s (u, v, t, p), the density and C5 (boundary.f90:62-70,
+// 157-162, 321-326).
 template <class R>
+__device__ __forceinline__ void ghost_complete(Prim<R> &s, R rho, R c5, const Consts<R> &K) {
+    ghost_fill(s, rho, K);
+    s.c[4] = c5;
+    s.f[4] = s.c[4] * s.u;
+    s.g[4] = s.c[4] * s.v;
+}
+
+// Type-6 ghosts come from elsewhere (a port-column walk): callers that meet
+// them pass a loader g6(u, v, t, p, rho, c5); with this one (grids without
+// type 6, or callers that handle it themselves) the last branch is type 7.
+struct NoBc6 {
+    template <class R>
+    __device__ void operator()(R &u, R &v, R &t, R &p, R &rho, R &c5) const {
+        u = v = t = p = rho = c5 = R(0);
+    }
+};
+
+// Ghost-cell state for boundary types 2, 3, 4, 5, 7 (boundary.f90:53-122, 310-327),
+// and 6 through g6.  par: parent state; (uo, vo, to, po): primitives of the
+// parent's opposite neighbour (type 3 only); ycc_g: ghost centroid y (type 7 only).
+template <class R, class G6 = NoBc6>
 __device__ __forceinline__ void ghost_state(Prim<R> &s, int type, const Prim<R> &par, R uo, R vo,
-                                            R to, R po, R ycc_g, const Consts<R> &K) {
+                                            R to, R po, R ycc_g, const Consts<R> &K,
+                                            G6 g6 = G6{}) {
     if (type == 2) {                       // subsonic isentropic inlet
         s.u = fabs(par.u);
         s.v = -par.v;
@@ -184,11 +212,7 @@ __device__ __forceinline__ void ghost_state(Prim<R> &s, int type, const Prim<R> 
         R rho = tpow(K.s0 / (K.rr * s.t), K.inv_1mg);
         s.p = rho * K.rr * s.t;
         R amu = K.suth_sqrt_amol * tpow(s.t, R(0.6));
-        R rt = amu / rho;
-        ghost_fill(s, rho, K);
-        s.c[4] = rho * rt;
-        s.f[4] = s.c[4] * s.u;
-        s.g[4] = s.c[4] * s.v;
+        ghost_complete(s, rho, rho * (amu / rho), K);
     } else if (type == 3) {                // extrapolation "farfield"
         s.u = fabs(R(2) * par.u - uo);
         s.v = R(2) * par.v - vo;
@@ -231,22 +255,68 @@ __device__ __forceinline__ void ghost_state(Prim<R> &s, int type, const Prim<R> 
         s.c[4] = -par.c[4];
         s.f[4] = -par.f[4];
         s.g[4] = -par.g[4];
-    } else {                               // 7: 1/7-power boundary-layer inlet
-        s.p = K.pinf;
-        s.t = K.tinf;
-        s.v = -par.v;
-        R rho = s.p / (K.rr * s.t);
-        if (ycc_g <= K.deltabl)
-            s.u = K.bl_u * tpow(ycc_g / K.deltabl, R(kBlExponent)) - par.u;
-        else
-            s.u = K.bl_u - par.u;
-        R amu = K.suth_sqrt_amol * tpow(s.t, R(0.6));
-        R rt = amu / rho;
-        ghost_fill(s, rho, K);
-        s.c[4] = rho * rt;
-        s.f[4] = s.c[4] * s.u;
-        s.g[4] = s.c[4] * s.v;
+    } else {                               // type 7, and 6 when g6 is a loader
+        R rho, c5;
+        if (std::is_same<G6, NoBc6>::value || type == 7) {   // 1/7-power boundary-layer inlet
+            s.p = K.pinf;
+            s.t = K.tinf;
+            s.v = -par.v;
+            rho = s.p / (K.rr * s.t);
+            if (ycc_g <= K.deltabl)
+                s.u = K.bl_u * tpow(ycc_g / K.deltabl, R(kBlExponent)) - par.u;
+            else
+                s.u = K.bl_u - par.u;
+            R amu = K.suth_sqrt_amol * tpow(s.t, R(0.6));
+            c5 = rho * (amu / rho);
+        } else {                           // 6: burning surface, formed beforehand
+            g6(s.u, s.v, s.t, s.p, rho, c5);
+        }
+        ghost_complete(s, rho, c5, K);
     }
+}
+
+// BC type 6 (boundary.f90:128-305): the ghost of a burning propellant surface.
+// The port mass flow GPORT sums C1*U*DELY (*YCC when IAXIS = 1) over the column
+// of cells from the parent across to the far boundary, entries [e0, e1) of
+// cells / dely in the reference's order; cell(n, c1, c2, ycc) loads a column
+// cell's C1, C2 and centroid y.  (scx, scy): the parent's surface vector on the
+// ghost's face; dhyd = |YCC(ghost) + YCC(parent)|.  The burn rate follows
+// ACOMB (P CONVKSC)^ANCOMB with the erosive correction above GVAL = GTH.
+template <class R, class Cell>
+__device__ __forceinline__ void bc6_state(Prim<R> &s, const Prim<R> &par, R scx, R scy, R dhyd,
+                                          int e0, int e1, const int *cells, const double *dely,
+                                          Cell cell, const Consts<R> &K) {
+    R gp = R(0);
+    for (int e = e0; e < e1; ++e) {
+        R c1, c2, yc;
+        cell(cells[e], c1, c2, yc);
+        const R u = Recip<R>(c1).div(c2);          // U = C2/C1 (COMPUTE_PRIMITIVE_VARS)
+        const R t = K.iaxis == 1 ? c1 * u * (R)dely[e] * yc : c1 * u * (R)dely[e];
+        gp = e == e0 ? t : gp + t;
+    }
+    gp = K.iaxis == 1 ? fdiv(R(2) * gp, dhyd * dhyd) : fdiv(gp, dhyd);
+    const R dnr = tsqrt(scx * scx + scy * scy);
+    const R enx = fdiv(-scx, dnr), eny = fdiv(-scy, dnr);
+    R rho = par.c[0];
+    const R rdot = K.acomb * tpow(par.p * R(kConvKsc), K.ancomb);
+    R rdoteb = rdot;
+    R vel = fdiv(K.rhoprop, rho) * rdoteb;
+    s.p = par.p;
+    s.u = vel * enx + vel * enx - par.u;
+    s.v = vel * eny + vel * eny - par.v;
+    s.t = K.tflame - fdiv(R(0.5) * (s.u * s.u + s.v * s.v), K.cp);
+    R amu = K.suth_sqrt_amol * tpow(s.t, R(0.6));
+    const R gval = fdiv(gp, K.rhoprop * rdot) *
+                   tpow(fdiv(K.rhoprop * rdot * dhyd, amu), R(-0.125));
+    if (gval > R(kGth)) rdoteb = rdoteb * (R(1) + R(0.023) * (tpow(gval, R(0.8)) - K.gth_08));
+    vel = fdiv(K.rhoprop, rho) * rdoteb;
+    s.u = vel * enx + vel * enx - par.u;
+    s.v = vel * eny + vel * eny - par.v;
+    s.t = K.tflame - fdiv(R(0.5) * (s.u * s.u + s.v * s.v), K.cp);
+    rho = fdiv(s.p, K.rr * s.t);
+    amu = K.suth_sqrt_amol * tpow(s.t, R(0.6));
+    const R rt = fdiv(amu, rho);
+    ghost_complete(s, rho, rho * rt, K);
 }
 
 // One face of VISCOUS_FACE (viscous.f90:130-217).  A = cell, B/D = face nodes,

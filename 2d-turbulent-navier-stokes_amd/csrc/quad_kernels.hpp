@@ -18,7 +18,7 @@ constexpr int kCellsPerBlock = kBlock / 4;
 // ---------------------------------------------------------------------------------
 // kq_pre: blocks [0, nbq_cells) do cells, the rest do nodes (64 per block); the
 // last block folds the previous iteration's monitor partials.
-template <class SD, class R>
+template <class SD, class R, bool B6>
 __global__ void __launch_bounds__(kBlock) kq_pre(FDev F, Consts<double> K, Consts<R> KR,
                                                  int nbq_cells, int nb_partials) {
     const int lane = threadIdx.x & 3;
@@ -97,8 +97,12 @@ __global__ void __launch_bounds__(kBlock) kq_pre(FDev F, Consts<double> K, Const
                 uvtp_from_q<R>((R)Q[o], (R)Q[nt + o], (R)Q[2 * nt + o], (R)Q[3 * nt + o], uo, vo,
                                to, po, KR);
             }
-            ghost_state<R>(s, ty, pp, uo, vo, to, po, (R)F.ycc[id], KR);
-            uu = s.u; vv = s.v; tt = s.t; rho = s.c[0]; c5 = s.c[4];
+            if (B6 && ty == 6) {
+                bc6_load<R>(F, k, uu, vv, tt, rho, c5);
+            } else {
+                ghost_state<R>(s, ty, pp, uo, vo, to, po, (R)F.ycc[id], KR);
+                uu = s.u; vv = s.v; tt = s.t; rho = s.c[0]; c5 = s.c[4];
+            }
         }
         const R rr = fdiv(c5, rho);
 #define TDT_NODE_ACC(K_)                                                           \
@@ -130,7 +134,7 @@ __global__ void __launch_bounds__(kBlock) kq_pre(FDev F, Consts<double> K, Const
 // ---------------------------------------------------------------------------------
 // kq_stage: one RK stage; stage 1 also forms the frozen dissipation D and
 // viscous fluxes VIS (solver_state.f90:120-223, viscous.f90:62-217).
-template <class R, class SI, class SO, class SD, int STAGE>
+template <class R, class SI, class SO, class SD, int STAGE, bool B6>
 __global__ void __launch_bounds__(kBlock) kq_stage(FDev F, Consts<double> K, Consts<R> KR,
                                                    double alpha) {
     const int lane = threadIdx.x & 3;
@@ -154,7 +158,9 @@ __global__ void __launch_bounds__(kBlock) kq_stage(FDev F, Consts<double> K, Con
     // the parent's opposite neighbour, needed by type-3 ghosts, is the neighbour
     // the lane across the cell just loaded
     const R uo = qopp(pj.u), vo = qopp(pj.v), to = qopp(pj.t), po = qopp(pj.p);
-    if (ghost) ghost_state<R>(pj, F.gtype[j - n], pi, uo, vo, to, po, (R)F.ycc[j], KR);
+    if (ghost)
+        ghost_state<R>(pj, F.gtype[j - n], pi, uo, vo, to, po, (R)F.ycc[j], KR,
+                       g6_for<B6, R>(F, j - n));
 
     // central flux residual (residual.f90:34-54): face terms summed in face order
     const R sx = (R)F.sc[(2 * lane) * n + i], sy = (R)F.sc[(2 * lane + 1) * n + i];

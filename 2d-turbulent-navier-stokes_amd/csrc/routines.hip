@@ -226,8 +226,19 @@ __global__ void __launch_bounds__(kBlock) k_bc(Dev D, Consts<double> K) {
         int par = D.gparent[k], opp = D.gopp[k], gg = D.n + k;
         Prim<double> pp, s;
         load_prim(D, par, pp);
-        ghost_state(s, D.gtype[k], pp, D.u[opp], D.v[opp], D.t[opp], D.p[opp], D.ycc[gg], K);
         const int nt = D.nt;
+        if (D.gtype[k] == 6) {
+            const int f = D.gside[k] - 1, n = D.n;
+            bc6_state(s, pp, D.sc[(2 * f) * n + par], D.sc[(2 * f + 1) * n + par],
+                      fabs(D.ycc[gg] + D.ycc[par]), D.g6ptr[k], D.g6ptr[k + 1], D.g6cell,
+                      D.g6dely,
+                      [&](int c, double &c1, double &c2, double &yc) {
+                          c1 = D.c[c]; c2 = D.c[nt + c]; yc = D.ycc[c];
+                      },
+                      K);
+        } else {
+            ghost_state(s, D.gtype[k], pp, D.u[opp], D.v[opp], D.t[opp], D.p[opp], D.ycc[gg], K);
+        }
         for (int m = 0; m < 5; ++m) {
             D.c[m * nt + gg] = s.c[m];
             D.f[m * nt + gg] = s.f[m];

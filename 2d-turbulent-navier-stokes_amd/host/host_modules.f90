@@ -28,6 +28,7 @@ MODULE SOLVER_STATE
     INTEGER :: NODS, NELES, NGHOSTS, NMAX, LOCAL, NITER, ISTART, IAXIS, NPSEUDO, NITEROLD
     DOUBLE PRECISION :: GAMA, AMOL, CFL, PRANDTL, PRANDTLT, P0, T0, PAMB, AMACHINIT, RTRED
     DOUBLE PRECISION :: RINIT, PINF, AMINF, DELTABL, RR
+    DOUBLE PRECISION :: TFLAME, ANCOMB, ACOMB, RHOPROP
     ! host copies, allocated to the sizes grid.in / bc.in give (no NN limit)
     DOUBLE PRECISION, ALLOCATABLE, TARGET :: X(:), Y(:)
     INTEGER(C_INT32_T), ALLOCATABLE, TARGET :: NOD(:,:), NC(:,:)
@@ -128,6 +129,7 @@ CONTAINS
         PRM%PRANDTLT = PRANDTLT; PRM%P0 = P0; PRM%T0 = T0; PRM%PAMB = PAMB
         PRM%AMACHINIT = AMACHINIT; PRM%RTRED = RTRED
         PRM%PINF = PINF; PRM%AMINF = AMINF; PRM%DELTABL = DELTABL
+        PRM%TFLAME = TFLAME; PRM%ANCOMB = ANCOMB; PRM%ACOMB = ACOMB; PRM%RHOPROP = RHOPROP
         PRM%LOCAL = LOCAL; PRM%NPSEUDO = NPSEUDO; PRM%IAXIS = IAXIS; PRM%RESERVED = 0
         CALL TDT_CHECK(TDT_CREATE(CTX, M, PRM, PRECISION_MODE, TDT_ORDER_AUTO), C_NULL_PTR, &
                        'COMPUTE_GEOMETRY')

@@ -17,6 +17,7 @@ MODULE TDT_C
     TYPE, BIND(C) :: TDT_PARAMS_T
         REAL(C_DOUBLE) :: GAMA, AMOL, CFL, PRANDTL, PRANDTLT, P0, T0, PAMB, AMACHINIT, RTRED
         REAL(C_DOUBLE) :: PINF, AMINF, DELTABL
+        REAL(C_DOUBLE) :: TFLAME, ANCOMB, ACOMB, RHOPROP
         INTEGER(C_INT32_T) :: LOCAL, NPSEUDO, IAXIS, RESERVED
     END TYPE TDT_PARAMS_T
 

@@ -10,8 +10,9 @@
 ! CALL order, so the arrays POSTPROCESS reads are those of the reference's
 ! last stage.  TDT_HOST_MODE=routines runs every iteration that way.
 ! TDT_PRECISION=fp32-mixed selects the mixed-precision path.
-! Optional bc7.in: PINF AMINF DELTABL, the BC type-7 inputs the reference
-! never assigns (they stay 0 without it, as in the reference).
+! Optional bc7.in: PINF AMINF DELTABL, and bc6.in: TFLAME ANCOMB ACOMB RHOPROP,
+! the BC type-7 and type-6 inputs the reference never assigns (they stay 0
+! without them, as in the reference).
 !===============================================================================
 PROGRAM TURBNS
     USE, INTRINSIC :: ISO_C_BINDING
@@ -26,7 +27,7 @@ PROGRAM TURBNS
     IMPLICIT NONE
     INTEGER :: I, ITER, CHUNK, IOS
     INTEGER(C_INT32_T) :: BAD
-    LOGICAL :: ROUTINES, HAVE_BL
+    LOGICAL :: ROUTINES, HAVE_BL, HAVE_B6
     CHARACTER(LEN=32) :: ENVV
     DOUBLE PRECISION, ALLOCATABLE :: L2H(:,:), LINFH(:,:)
     DOUBLE PRECISION :: L2(5), LINF(5)
@@ -42,6 +43,14 @@ PROGRAM TURBNS
         READ(27, *, IOSTAT=IOS) PINF, AMINF, DELTABL
         CLOSE(27)
         IF (IOS .NE. 0) STOP 'bc7.in: expected PINF AMINF DELTABL'
+    END IF
+    TFLAME = 0.0D0; ANCOMB = 0.0D0; ACOMB = 0.0D0; RHOPROP = 0.0D0
+    INQUIRE(FILE='bc6.in', EXIST=HAVE_B6)
+    IF (HAVE_B6) THEN
+        OPEN(27, FILE='bc6.in')
+        READ(27, *, IOSTAT=IOS) TFLAME, ANCOMB, ACOMB, RHOPROP
+        CLOSE(27)
+        IF (IOS .NE. 0) STOP 'bc6.in: expected TFLAME ANCOMB ACOMB RHOPROP'
     END IF
     CALL GET_ENVIRONMENT_VARIABLE('TDT_HOST_MODE', ENVV)
     ROUTINES = TRIM(ENVV) .EQ. 'routines'

@@ -50,7 +50,8 @@ class _Mesh(ctypes.Structure):
 class _Params(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in (
         "gama", "amol", "cfl", "prandtl", "prandtlt", "p0", "t0", "pamb", "amachinit", "rtred",
-        "pinf", "aminf", "deltabl")] + [(n, ctypes.c_int32) for n in (
+        "pinf", "aminf", "deltabl", "tflame", "ancomb", "acomb", "rhoprop")] + [
+            (n, ctypes.c_int32) for n in (
             "local", "npseudo", "iaxis", "reserved")]
 
 
@@ -111,6 +112,7 @@ def _i(a):
 def make_params(flow: Flow) -> _Params:
     return _Params(flow.gama, flow.amol, flow.cfl, flow.prandtl, flow.prandtlt, flow.p0, flow.t0,
                    flow.pamb, flow.amachinit, flow.rtred, flow.pinf, flow.aminf, flow.deltabl,
+                   flow.tflame, flow.ancomb, flow.acomb, flow.rhoprop,
                    int(flow.local), int(flow.npseudo), int(flow.iaxis), 0)
 
 

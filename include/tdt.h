@@ -63,11 +63,14 @@ typedef struct tdt_mesh {
     const int32_t *parent, *ghost, *type, *side; /* [nghosts] bc.in columns          */
 } tdt_mesh;
 
-/* flow.in fields (turbnstwod.f90:20-23) and the BC type-7 inputs PINF, AMINF,
- * DELTABL that the reference reads but never assigns (boundary.f90:312-317). */
+/* flow.in fields (turbnstwod.f90:20-23) and the inputs the reference reads but
+ * never assigns: BC type 7 PINF, AMINF, DELTABL (boundary.f90:312-317) and BC
+ * type 6 TFLAME, ANCOMB, ACOMB, RHOPROP (flame temperature, burn-rate exponent
+ * and coefficient, propellant density; boundary.f90:138-153). */
 typedef struct tdt_params {
     double gama, amol, cfl, prandtl, prandtlt, p0, t0, pamb, amachinit, rtred;
     double pinf, aminf, deltabl;
+    double tflame, ancomb, acomb, rhoprop;
     int32_t local, npseudo, iaxis, reserved;
 } tdt_params;
 

@@ -64,7 +64,8 @@ def test_fused_matches_reference(golden, case):
     assert worst <= REL, f"final fields relative L2 errors {errs}"
 
 
-@pytest.mark.parametrize("case", ["t2d_chan", "t2d_ramp", "taxi_duct", "t7_plate", "case1_axi"])
+@pytest.mark.parametrize("case", ["t2d_chan", "t2d_ramp", "taxi_duct", "t7_plate", "t6_port_axi",
+                                  "case1_axi"])
 def test_fused_equals_routine_path(golden, case, monkeypatch):
     """The fused kernels and the routine-level kernels compute the same
     expressions in the same order: the states must be bitwise identical, and
@@ -168,7 +169,7 @@ def test_fp32_mixed_tracks_fp64(golden):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32-mixed"])
-@pytest.mark.parametrize("case", ["t2d_ramp", "taxi_duct", "t7_plate", "case1_axi"])
+@pytest.mark.parametrize("case", ["t2d_ramp", "taxi_duct", "t7_plate", "t6_port_axi", "case1_axi"])
 def test_kernel_sets_identical(golden, case, precision, monkeypatch):
     """The merged, split and quad kernel sets evaluate the same expressions in
     the same order.  FP64: states bitwise identical, and the monitor norms too

@@ -76,7 +76,7 @@ def _create(mesh, flow):
 
 @pytest.mark.parametrize("mutate,code", [
     (lambda m: m.ghost.__setitem__(0, 3), lib.TDT_ERR_MESH),          # collides with a cell id
-    (lambda m: m.btype.__setitem__(0, 6), lib.TDT_ERR_UNSUPPORTED),   # combustion boundary
+    (lambda m: m.btype.__setitem__(0, 1), lib.TDT_ERR_MESH),          # unknown type
     (lambda m: m.btype.__setitem__(0, 9), lib.TDT_ERR_MESH),          # unknown type
     (lambda m: m.side.__setitem__(0, 3), lib.TDT_ERR_MESH),           # wrong face
     (lambda m: m.nc.__setitem__((1, 0), 10**6), lib.TDT_ERR_MESH),    # dangling neighbour
@@ -89,6 +89,16 @@ def test_mesh_validation(golden, mutate, code):
     with pytest.raises(tdt.TdtError) as ei:
         _create(m, g.flow)
     assert ei.value.code == code, str(ei.value)
+
+
+def test_bc6_column_must_cross_a_cell():
+    """A burning (type-6) face whose parent's opposite neighbour is a ghost (a
+    one-cell-thick domain) has no port column to sum: refused."""
+    m = gridgen.channel(nx=4, ny=1, bc={"left": 5, "right": 3, "bottom": 6, "top": 4})
+    f = gridgen.flow_channel(tflame=2500.0, ancomb=0.35, acomb=1e-3, rhoprop=1700.0)
+    with pytest.raises(tdt.TdtError) as ei:
+        _create(m, f)
+    assert ei.value.code == lib.TDT_ERR_UNSUPPORTED, str(ei.value)
 
 
 def test_minus_one_neighbours_resolve_through_bc(golden):
