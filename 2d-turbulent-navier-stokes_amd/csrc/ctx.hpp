@@ -51,6 +51,25 @@ struct Dev {
 
 struct FusedBufs;  // fused.hip
 
+// A batch of independent cases in one context (tdt_batch_create).  Cells of
+// case c are internal ids [cell_off[c], cell_off[c] + ncell[c]), in the order
+// tdt_create would give that case alone; its ghosts and its grid.in ids follow
+// the same pattern.  Cell and node blocks never straddle two cases.
+struct BatchInfo {
+    int ncases = 0;
+    std::vector<int> cell_off, ncell, ghost_off, nghost;
+    std::vector<int> orig_cell_off;          // first combined original cell id of each case
+    std::vector<tdt_params> params;
+    std::vector<double> rinit;
+    std::vector<Consts<double>> k64;
+    std::vector<Consts<float>> kf;
+    std::vector<int4> cblk, nblk, ccase;     // cblk / nblk padded to multiples of 8 blocks
+    int nb_cells = 0, nb_nodes = 0;          // blocks holding cells / nodes
+    Consts<double> *d_k64 = nullptr;
+    Consts<float> *d_kf = nullptr;
+    int4 *d_cblk = nullptr, *d_nblk = nullptr, *d_ccase = nullptr;
+};
+
 }  // namespace tdt
 
 struct tdt_ctx {
@@ -71,6 +90,7 @@ struct tdt_ctx {
     double ktimes[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     int geom_version = 0;     // bumped when tdt_set_array changes geometry
     double *post = nullptr;   // [6][m] POSTPROCESS output, allocated on first use
+    tdt::BatchInfo *batch = nullptr;   // tdt_batch_create contexts only
 };
 
 namespace tdt {

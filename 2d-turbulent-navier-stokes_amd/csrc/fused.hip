@@ -30,13 +30,24 @@
 namespace tdt {
 
 hipError_t enqueue_fp64(tdt_ctx *c, FusedBufs &B) {
-    return B.F.has_bc6 ? enqueue_iteration<double, double, double, true>(c, B, c->K)
-                       : enqueue_iteration<double, double, double, false>(c, B, c->K);
+    if (B.F.ncases) return enqueue_iteration<double, double, double, false, true>(c, B, c->K);
+    return B.F.has_bc6 ? enqueue_iteration<double, double, double, true, false>(c, B, c->K)
+                       : enqueue_iteration<double, double, double, false, false>(c, B, c->K);
 }
 
 namespace {
 
 __global__ void __launch_bounds__(kBlock) k_finalize(FDev F, int nblk) { fold_monitor(F, nblk); }
+
+// Batch: the last iteration's partials, one block per case, then the counter.
+__global__ void __launch_bounds__(kBlock) k_finalize_batch(FDev F) { fold_case(F, blockIdx.x); }
+
+__global__ void k_batch_tick(int *ctr) {
+    if (ctr[3]) {
+        ctr[0] += 1;
+        ctr[3] = 0;
+    }
+}
 
 __global__ void k_reset_counters(int *ctr, unsigned long long *dtslot) {
     ctr[0] = 0;
@@ -120,6 +131,19 @@ int fused_setup(tdt_ctx *c) {
     B->nb_cells = (int)nblocks(D.n);
     F.ncb8 = (B->nb_cells + 7) & ~7;
     B->nb_nodes = (int)nblocks(D.m);
+    if (c->batch) {
+        // block tables: every block holds cells (nodes) of one case
+        const BatchInfo &bi = *c->batch;
+        F.ncases = bi.ncases;
+        F.cblk = bi.d_cblk;
+        F.nblk = bi.d_nblk;
+        F.ccase = bi.d_ccase;
+        F.kc64 = bi.d_k64;
+        F.kcr = c->precision == TDT_FP64 ? (const void *)bi.d_k64 : (const void *)bi.d_kf;
+        B->nb_cells = bi.nb_cells;
+        F.ncb8 = (int)bi.cblk.size();
+        B->nb_nodes = bi.nb_nodes;
+    }
     B->nbq_cells = (D.n + kBlock / 4 - 1) / (kBlock / 4);
     B->nbq_nodes = (D.m + kBlock / 4 - 1) / (kBlock / 4);
     // TDT_KERNELS=merged|split|quad (or a kernel-set number) pins the kernel
@@ -136,6 +160,7 @@ int fused_setup(tdt_ctx *c) {
             B->tuned = true;
         }
     }
+    if (c->batch) B->kset &= kSt1Split;   // no quad form for a batch
     // fused-only neighbour and node-gather tables (built from the prepared mesh)
     const PreparedMesh &pm = c->pm;
     if (D.nt > kIdMask) {
@@ -281,26 +306,28 @@ static hipError_t autotune(tdt_ctx *c, FusedBufs &B) {
     const int prof = c->profiling;
     hipError_t e = hipMemcpyAsync(B.qbak, F.qn, qbytes, hipMemcpyDeviceToDevice, c->stream);
     const int sets[3] = {kSetMerged, kSetSplit, kSetQuad};
+    const int nsets = F.ncases ? 2 : 3;                // a batch has no quad form
     double t[3][kNumFusedKernels] = {};
-    for (int k = 0; k < 3 && e == hipSuccess; ++k) {
+    for (int k = 0; k < nsets && e == hipSuccess; ++k) {
         e = time_set(c, B, sets[k], kTuneIterations, t[k], nullptr);
         if (e == hipSuccess)
             e = hipMemcpyAsync(F.qn, B.qbak, qbytes, hipMemcpyDeviceToDevice, c->stream);
     }
     // per slot: pre-pass (merged's k_pre vs quad's kq_pre), stage 1 (merged,
     // split = k_dv + stage 1, quad), stages 2-4 (merged's vs quad's)
-    int comp = t[2][0] < t[0][0] ? kPreQuad : 0;
+    int comp = nsets > 2 && t[2][0] < t[0][0] ? kPreQuad : 0;
     const double s1[3] = {t[0][2], t[1][1] + t[1][2], t[2][2]};
     const int s1set[3] = {0, kSt1Split, kSt1Quad};
     int b1 = 0;
-    for (int k = 1; k < 3; ++k) b1 = s1[k] < s1[b1] ? k : b1;
+    for (int k = 1; k < nsets; ++k) b1 = s1[k] < s1[b1] ? k : b1;
     comp |= s1set[b1];
-    if (t[2][3] + t[2][4] + t[2][5] < t[0][3] + t[0][4] + t[0][5]) comp |= kStQuad;
+    if (nsets > 2 && t[2][3] + t[2][4] + t[2][5] < t[0][3] + t[0][4] + t[0][5]) comp |= kStQuad;
     // whole iterations: the composed set against the uniform ones
     const int cand[4] = {kSetMerged, kSetSplit, kSetQuad, comp};
     int best = kSetMerged;
     float best_ms = 3.4e38f;
     for (int k = 0; k < 4 && e == hipSuccess; ++k) {
+        if (k == 2 && nsets < 3) continue;
         if (k == 3 && (comp == kSetMerged || comp == kSetSplit || comp == kSetQuad)) break;
         float ms = 0.f;
         e = time_set(c, B, cand[k], kTuneIterations, nullptr, &ms);
@@ -339,9 +366,10 @@ int fused_iterate(tdt_ctx *c, int niter, double *l2, double *linf, int *first_no
         }
         B.geom_version = c->geom_version;
     }
+    const size_t nrec = F.ncases ? (size_t)F.ncases : 1;   // history records per iteration
     if ((size_t)niter > B.hist_cap) {
         size_t cap = std::max((size_t)niter, (size_t)1024);
-        double *h = dalloc<double>(B, cap * 10);
+        double *h = dalloc<double>(B, cap * 10 * nrec);
         if (!h) {
             c->err = "device allocation of the monitor history failed";
             return TDT_ERR_DEVICE;
@@ -392,9 +420,14 @@ int fused_iterate(tdt_ctx *c, int niter, double *l2, double *linf, int *first_no
             ++done;
         }
     }
-    if (e == hipSuccess) {
+    if (e == hipSuccess && F.ncases) {
+        hipLaunchKernelGGL(k_finalize_batch, dim3(F.ncases), dim3(kBlock), 0, c->stream, F);
+        hipLaunchKernelGGL(k_batch_tick, dim3(1), dim3(1), 0, c->stream, F.ctr);
+    } else if (e == hipSuccess) {
         hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, c->stream, F,
                            partial_blocks(B, B.kset));
+    }
+    if (e == hipSuccess) {
         hipLaunchKernelGGL(k_cold_from_qn, dim3(nblocks(F.n)), dim3(kBlock), 0, c->stream, F,
                            c->D.cold);
     }
@@ -403,7 +436,7 @@ int fused_iterate(tdt_ctx *c, int niter, double *l2, double *linf, int *first_no
     if (e == hipSuccess) e = hipMemcpyAsync(ctr, F.ctr, sizeof ctr, hipMemcpyDeviceToHost, c->stream);
     std::vector<double> h;
     if (e == hipSuccess && (l2 || linf) && niter > 0) {
-        h.resize((size_t)niter * 10);
+        h.resize((size_t)niter * 10 * nrec);
         e = hipMemcpyAsync(h.data(), F.hist, h.size() * sizeof(double), hipMemcpyDeviceToHost,
                            c->stream);
     }
@@ -412,10 +445,11 @@ int fused_iterate(tdt_ctx *c, int niter, double *l2, double *linf, int *first_no
         c->err = std::string("fused iteration: ") + hipGetErrorString(e);
         return TDT_ERR_DEVICE;
     }
-    for (int it = 0; it < (int)h.size() / 10; ++it)
+    // records [iteration][case]: L2 in 0..4, Linf in 5..9
+    for (size_t r = 0; r < h.size() / 10; ++r)
         for (int q = 0; q < 5; ++q) {
-            if (l2) l2[(size_t)it * 5 + q] = h[(size_t)it * 10 + q];
-            if (linf) linf[(size_t)it * 5 + q] = h[(size_t)it * 10 + 5 + q];
+            if (l2) l2[r * 5 + q] = h[r * 10 + q];
+            if (linf) linf[r * 5 + q] = h[r * 10 + 5 + q];
         }
     if (first_nonfinite) *first_nonfinite = ctr[2];
     if (ctr[0] != niter) {

@@ -59,6 +59,13 @@ struct FDev {
     double *hist;                   // [niter][10]
     int *ctr;                       // 0: iteration, 2: first non-finite iteration,
                                     // 3: stage-4 monitor partials pending
+    // batch of independent cases (ncases > 0): each logical cell / node block
+    // holds cells / nodes of one case, {first, count, case, 0}; ccase[c] =
+    // {first logical cell block, cell blocks, cells, 0}; kc64 / kcr: the cases'
+    // flow constants (Consts<double>, Consts<R>)
+    int ncases;
+    const int4 *cblk, *nblk, *ccase;
+    const void *kc64, *kcr;
 };
 
 struct FusedBufs {

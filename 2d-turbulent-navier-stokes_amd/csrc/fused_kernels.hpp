@@ -58,6 +58,41 @@ __device__ __forceinline__ void fold_monitor(const FDev &F, int nblk) {
     }
 }
 
+// Batch: folds case cs's pending partials (one block per case) into
+// hist[it][cs].  The iteration counter moves on in the next stage 4 (or
+// k_batch_tick), after every case has read it.
+__device__ __forceinline__ void fold_case(const FDev &F, int cs) {
+    if (F.ctr[3] == 0) return;   // uniform across the block
+    __shared__ int flag;
+    if (threadIdx.x == 0) flag = 0;
+    __syncthreads();
+    const int it = F.ctr[0];
+    const int4 r = F.ccase[cs];
+    reduce_partials(F.part + (size_t)r.x * kPartStride, r.y, r.z,
+                    F.hist + ((size_t)it * F.ncases + cs) * 10, &flag);
+    __syncthreads();
+    if (threadIdx.x == 0 && flag) atomicCAS(&F.ctr[2], 0, it + 1);
+}
+
+// Batch: the flow constants of case cs in place of the kernel arguments'.  The
+// cases of a batch share the gas and model constants (checked when the batch is
+// made), so only the fields of the case's state and boundary data are loaded.
+template <class R>
+__device__ __forceinline__ void case_fields(Consts<R> &K, const Consts<R> &C) {
+    K.t0 = C.t0; K.s0 = C.s0; K.pamb = C.pamb; K.pamb_set = C.pamb_set;
+    K.pinf = C.pinf; K.tinf = C.tinf; K.bl_u = C.bl_u; K.deltabl = C.deltabl;
+    K.rtol = C.rtol; K.rtol_rtred = C.rtol_rtred;
+}
+
+template <bool BT, class R>
+__device__ __forceinline__ void case_consts(const FDev &F, int cs, Consts<double> &K,
+                                            Consts<R> &KR) {
+    if constexpr (BT) {
+        case_fields(K, static_cast<const Consts<double> *>(F.kc64)[cs]);
+        case_fields(KR, static_cast<const Consts<R> *>(F.kcr)[cs]);
+    }
+}
+
 // BC type 6 ghost k (parent par) from the cells' states in q (storage type T,
 // component stride qs): the port column's C1, C2, the parent face's surface
 // vector and the centroids, all as the flux arithmetic reads them.
@@ -183,20 +218,37 @@ __device__ __forceinline__ void node_gather_list(const FDev &F, const Consts<R> 
 // to multiples of 8 and spread so that each XCD works on contiguous cells and
 // the nodes around them), and a last block of its own that folds the previous
 // iteration's monitor partials (off every cell's path).
-template <class SD, class R, bool B6>
+template <class SD, class R, bool B6, bool BT>
 __global__ void __launch_bounds__(kBlock) k_pre(FDev F, Consts<double> K, Consts<R> KR,
                                                 int nb_partials, int ncb8, int nnb8) {
     int b = blockIdx.x;
-    if (b == (int)gridDim.x - 1) {
+    if constexpr (BT) {          // one folding block per case
+        const int first_fold = (int)gridDim.x - F.ncases;
+        if (b >= first_fold) {
+            fold_case(F, b - first_fold);
+            return;
+        }
+    } else if (b == (int)gridDim.x - 1) {
         fold_monitor(F, nb_partials);
         return;
     }
     if (b < ncb8) {
         const int cb = F.remap ? xcd_chunk(b, ncb8) : b;
-        if (cb * kBlock >= F.n) return;
-        int i = cb * kBlock + threadIdx.x;
+        int i;
+        bool act;
+        if constexpr (BT) {
+            const int4 bl = F.cblk[cb];
+            if (bl.y == 0) return;
+            case_consts<BT>(F, bl.z, K, KR);
+            i = bl.x + threadIdx.x;
+            act = (int)threadIdx.x < bl.y;
+        } else {
+            if (cb * kBlock >= F.n) return;
+            i = cb * kBlock + threadIdx.x;
+            act = i < F.n;
+        }
         double dtl = 100000.0;
-        if (i < F.n) {
+        if (act) {
             const int nt = F.nt;
             const double *Q = F.qn;
             double c0 = Q[i], c1 = Q[nt + i], c2 = Q[2 * nt + i], c3 = Q[3 * nt + i],
@@ -237,8 +289,16 @@ __global__ void __launch_bounds__(kBlock) k_pre(FDev F, Consts<double> K, Consts
         return;
     }
     const int nbl = F.remap ? xcd_chunk(b - ncb8, nnb8) : b - ncb8;
-    int v = nbl * kBlock + threadIdx.x;
-    if (v >= F.m) return;
+    int v;
+    if constexpr (BT) {
+        const int4 bl = F.nblk[nbl];
+        if ((int)threadIdx.x >= bl.y) return;
+        case_consts<BT>(F, bl.z, K, KR);
+        v = bl.x + threadIdx.x;
+    } else {
+        v = nbl * kBlock + threadIdx.x;
+        if (v >= F.m) return;
+    }
     const int hr = F.nrel[v];
     const int cnt = hr & 15;
     R au = R(0), av = R(0), at = R(0), arho = R(0), ar = R(0);
@@ -308,10 +368,18 @@ __global__ void __launch_bounds__(kBlock) k_pre(FDev F, Consts<double> K, Consts
 // ---------------------------------------------------------------------------------
 // k_dv: dissipation D and viscous fluxes VIS from Q^n in a kernel of their own
 // (the split variant; the merged variant forms them inside stage 1).
-template <class SD, class R, bool B6>
+template <class SD, class R, bool B6, bool BT>
 __global__ void __launch_bounds__(kBlock) k_dv(FDev F, Consts<double> K, Consts<R> KR) {
-    int i = cell_block(F) * kBlock + threadIdx.x;
-    if (i >= F.n) return;
+    int i;
+    if constexpr (BT) {
+        const int4 bl = F.cblk[cell_block(F)];
+        if ((int)threadIdx.x >= bl.y) return;
+        case_consts<BT>(F, bl.z, K, KR);
+        i = bl.x + threadIdx.x;
+    } else {
+        i = cell_block(F) * kBlock + threadIdx.x;
+        if (i >= F.n) return;
+    }
     const int nt = F.nt, n = F.n, m = F.m;
     const double *Q = F.qn;
     double dti = K.local ? F.dt[i] : __longlong_as_double((long long)*F.dtslot);
@@ -417,17 +485,25 @@ __global__ void __launch_bounds__(kBlock) k_dv(FDev F, Consts<double> K, Consts<
 // its share of the source-term gradient differences.  SI / SO: stage input /
 // output storage types; SD: storage of D, VIS, sensors and nodal averages.
 // DV = false: stage 1 reads D and VIS written by k_dv (the split variant).
-template <class R, class SI, class SO, class SD, int STAGE, bool DV, bool B6>
+template <class R, class SI, class SO, class SD, int STAGE, bool DV, bool B6, bool BT>
 __global__ void __launch_bounds__(kBlock) k_rk(FDev F, Consts<double> K, Consts<R> KR,
                                                double alpha) {
     constexpr bool kInlineDV = STAGE == 1 && DV;
     const int lb = cell_block(F);
-    const int i = lb * kBlock + threadIdx.x;
+    int i, bcount = 0;
+    if constexpr (BT) {
+        const int4 bl = F.cblk[lb];
+        case_consts<BT>(F, bl.z, K, KR);
+        i = bl.x + threadIdx.x;
+        bcount = bl.y;
+    } else {
+        i = lb * kBlock + threadIdx.x;
+    }
     const int nt = F.nt, n = F.n;
     const void *qin = STAGE == 1 ? (const void *)F.qn : (const void *)F.st[STAGE & 1];
     const size_t sin = STAGE == 1 ? (size_t)nt : (size_t)n;
     double cnew[5] = {0, 0, 0, 0, 0}, cold[5] = {0, 0, 0, 0, 0};
-    const bool active = i < n;
+    const bool active = BT ? (int)threadIdx.x < bcount : i < n;
     if (active) {
         // neighbour ids (ghost ids carry their BC type in the top bits)
         const int4 nbt = F.nct[i];
@@ -623,9 +699,13 @@ __global__ void __launch_bounds__(kBlock) k_rk(FDev F, Consts<double> K, Consts<
         }
         // slot = logical block: the same cells, summed in the same order, as the
         // routine-level monitor's block partials
-        if (lb * kBlock < n)
+        if (BT ? bcount > 0 : lb * kBlock < n)
             block_partials_write(sq, mx, bad, F.part + (size_t)lb * kPartStride);
-        if (blockIdx.x == 0 && threadIdx.x == 0) F.ctr[3] = 1;   // partials pending
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            // batch: this iteration's pre-pass has folded the previous partials
+            if (BT && F.ctr[3]) F.ctr[0] += 1;
+            F.ctr[3] = 1;                                    // partials pending
+        }
     }
 }
 
@@ -650,10 +730,10 @@ void launch(tdt_ctx *c, FusedBufs &B, int slot, K kern, unsigned grid, Args... a
         hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, c->stream, args...);
 }
 
-template <class R, class SS, class SD, bool B6>
+template <class R, class SS, class SD, bool B6, bool BT>
 hipError_t enqueue_iteration(tdt_ctx *c, FusedBufs &B, const Consts<R> &KR) {
     const FDev &F = B.F;
-    const int ks = B.kset;
+    const int ks = BT ? B.kset & kSt1Split : B.kset;   // a batch has no quad form
     const unsigned nbc = F.ncb8, nq = B.nbq_cells;
     const int nparts = partial_blocks(B, ks);
     auto no_dv_slot = [&]() {   // slot 1 unused: dissipation and viscous live in stage 1
@@ -673,12 +753,14 @@ hipError_t enqueue_iteration(tdt_ctx *c, FusedBufs &B, const Consts<R> &KR) {
                nparts);
     } else {
         const int nnb8 = (B.nb_nodes + 7) & ~7;
-        launch(c, B, 0, k_pre<SD, R, B6>, F.ncb8 + nnb8 + 1, F, c->K, KR, nparts, F.ncb8, nnb8);
+        const unsigned nfold = BT ? F.ncases : 1;
+        launch(c, B, 0, k_pre<SD, R, B6, BT>, F.ncb8 + nnb8 + nfold, F, c->K, KR, nparts,
+               F.ncb8, nnb8);
     }
     switch (ks & kSt1Mask) {
     case kSt1Split:
-        launch(c, B, 1, k_dv<SD, R, B6>, nbc, F, c->K, KR);
-        launch(c, B, 2, k_rk<R, double, SS, SD, 1, false, B6>, nbc, F, c->K, KR, 0.25);
+        launch(c, B, 1, k_dv<SD, R, B6, BT>, nbc, F, c->K, KR);
+        launch(c, B, 2, k_rk<R, double, SS, SD, 1, false, B6, BT>, nbc, F, c->K, KR, 0.25);
         break;
     case kSt1Quad:
         no_dv_slot();
@@ -686,7 +768,7 @@ hipError_t enqueue_iteration(tdt_ctx *c, FusedBufs &B, const Consts<R> &KR) {
         break;
     default:
         no_dv_slot();
-        launch(c, B, 2, k_rk<R, double, SS, SD, 1, true, B6>, nbc, F, c->K, KR, 0.25);
+        launch(c, B, 2, k_rk<R, double, SS, SD, 1, true, B6, BT>, nbc, F, c->K, KR, 0.25);
     }
     if (ks & kStQuad) {
         bc6(k_bc6<R, SS, SD, 2>);
@@ -697,11 +779,11 @@ hipError_t enqueue_iteration(tdt_ctx *c, FusedBufs &B, const Consts<R> &KR) {
         launch(c, B, 5, kq_stage<R, SS, double, SD, 4, B6>, nq, F, c->K, KR, 1.0);
     } else {
         bc6(k_bc6<R, SS, SD, 2>);
-        launch(c, B, 3, k_rk<R, SS, SS, SD, 2, true, B6>, nbc, F, c->K, KR, 1.0 / 3.0);
+        launch(c, B, 3, k_rk<R, SS, SS, SD, 2, true, B6, BT>, nbc, F, c->K, KR, 1.0 / 3.0);
         bc6(k_bc6<R, SS, SD, 3>);
-        launch(c, B, 4, k_rk<R, SS, SS, SD, 3, true, B6>, nbc, F, c->K, KR, 0.5);
+        launch(c, B, 4, k_rk<R, SS, SS, SD, 3, true, B6, BT>, nbc, F, c->K, KR, 0.5);
         bc6(k_bc6<R, SS, SD, 4>);
-        launch(c, B, 5, k_rk<R, SS, double, SD, 4, true, B6>, nbc, F, c->K, KR, 1.0);
+        launch(c, B, 5, k_rk<R, SS, double, SD, 4, true, B6, BT>, nbc, F, c->K, KR, 1.0);
     }
     return hipGetLastError();
 }

@@ -8,8 +8,9 @@
 namespace tdt {
 
 hipError_t enqueue_mixed(tdt_ctx *c, FusedBufs &B) {
-    return B.F.has_bc6 ? enqueue_iteration<float, float, float, true>(c, B, c->Kf)
-                       : enqueue_iteration<float, float, float, false>(c, B, c->Kf);
+    if (B.F.ncases) return enqueue_iteration<float, float, float, false, true>(c, B, c->Kf);
+    return B.F.has_bc6 ? enqueue_iteration<float, float, float, true, false>(c, B, c->Kf)
+                       : enqueue_iteration<float, float, float, false, false>(c, B, c->Kf);
 }
 
 }  // namespace tdt

@@ -92,7 +92,8 @@ int order_bandwidth(int n, const std::vector<int> &nc_old, const std::vector<int
     return bw;
 }
 
-std::string prepare_mesh(const tdt_mesh &ms, int ordering, PreparedMesh &pm, int *code) {
+std::string prepare_mesh(const tdt_mesh &ms, int ordering, PreparedMesh &pm, int *code,
+                         const std::vector<int> *order) {
     *code = TDT_ERR_MESH;
     const int n = ms.neles, m = ms.nods, g = ms.nghosts;
     if (n <= 0 || m <= 0 || g < 0) return fmt("bad sizes nods=%ld neles=%ld nghosts=%ld", m, n, g);
@@ -164,10 +165,26 @@ std::string prepare_mesh(const tdt_mesh &ms, int ordering, PreparedMesh &pm, int
     std::vector<int> new2old = ident;
     pm.ordering_used = TDT_ORDER_INPUT;
     pm.bandwidth = bw_in;
-    const bool boundary_last = ordering == TDT_ORDER_BOUNDARY_LAST ||
-                               (ordering == TDT_ORDER_AUTO && n >= kBoundaryLastMinCells);
-    if (ordering == TDT_ORDER_RCM || ordering == TDT_ORDER_AUTO ||
-        ordering == TDT_ORDER_BOUNDARY_LAST) {
+    const bool boundary_last = !order && (ordering == TDT_ORDER_BOUNDARY_LAST ||
+                               (ordering == TDT_ORDER_AUTO && n >= kBoundaryLastMinCells));
+    if (order) {
+        std::vector<char> seen(n, 0);
+        bool perm = (int)order->size() == n;
+        for (int q = 0; perm && q < n; ++q) {
+            const int o = (*order)[q];
+            perm = o >= 0 && o < n && !seen[o];
+            if (perm) seen[o] = 1;
+        }
+        if (!perm) {
+            *code = TDT_ERR_ARG;
+            return "cell order is not a permutation of the cells";
+        }
+        new2old = *order;
+        std::vector<int> o2n(n);
+        for (int q = 0; q < n; ++q) o2n[new2old[q]] = q;
+        pm.bandwidth = order_bandwidth(n, nc_int, o2n);
+    } else if (ordering == TDT_ORDER_RCM || ordering == TDT_ORDER_AUTO ||
+               ordering == TDT_ORDER_BOUNDARY_LAST) {
         std::vector<int> rcm = rcm_order(n, nc_int), o2n(n);
         for (int q = 0; q < n; ++q) o2n[rcm[q]] = q;
         int bw_rcm = order_bandwidth(n, nc_int, o2n);

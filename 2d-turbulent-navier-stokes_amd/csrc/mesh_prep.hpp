@@ -46,8 +46,10 @@ struct PreparedMesh {
 };
 
 // Returns "" on success, otherwise an error message; *code receives the
-// TDT_ERR_* class of the failure.
-std::string prepare_mesh(const tdt_mesh &mesh, int ordering, PreparedMesh &out, int *code);
+// TDT_ERR_* class of the failure.  order (optional): the cell order to use,
+// new -> old 0-based, in place of the one `ordering` would choose.
+std::string prepare_mesh(const tdt_mesh &mesh, int ordering, PreparedMesh &out, int *code,
+                         const std::vector<int> *order = nullptr);
 
 // Bandwidth of a cell order: max |pos(i) - pos(j)| over interior neighbour pairs.
 int order_bandwidth(int n, const std::vector<int> &nc_old, const std::vector<int> &old2new);

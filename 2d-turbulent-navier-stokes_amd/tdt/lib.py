@@ -90,6 +90,12 @@ def load(path: str = LIB_PATH):
     L.tdt_kernel_times.argtypes = [vp, _dp]
     L.tdt_kernel_set.argtypes = [vp, _ip]
     L.tdt_postprocess.argtypes = [vp, _dp]
+    L.tdt_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int32, ctypes.POINTER(_Mesh),
+                                   ctypes.POINTER(_Params), ctypes.c_int32, ctypes.c_int32]
+    L.tdt_batch_cold_start.argtypes = [vp, _dp]
+    L.tdt_batch_set_state.argtypes = [vp, ctypes.c_int32, _dp, ctypes.c_double]
+    L.tdt_batch_get_state.argtypes = [vp, ctypes.c_int32, _dp]
+    L.tdt_batch_iterate.argtypes = [vp, ctypes.c_int32, _dp, _dp, _ip]
     _lib = L
     return L
 
@@ -116,6 +122,17 @@ def make_params(flow: Flow) -> _Params:
                    int(flow.local), int(flow.npseudo), int(flow.iaxis), 0)
 
 
+def _mesh_struct(mesh: Mesh, keep: list) -> _Mesh:
+    """The tdt_mesh view of a Mesh; the arrays it points to are appended to keep."""
+    k = [np.ascontiguousarray(a) for a in (
+        mesh.x_mm.astype(np.float64), mesh.y_mm.astype(np.float64), mesh.nod.astype(np.int32),
+        mesh.nc.astype(np.int32), mesh.parent.astype(np.int32), mesh.ghost.astype(np.int32),
+        mesh.btype.astype(np.int32), mesh.side.astype(np.int32))]
+    keep.extend(k)
+    return _Mesh(mesh.nods, mesh.neles, mesh.nghosts, _d(k[0]), _d(k[1]), _i(k[2]), _i(k[3]),
+                 _i(k[4]), _i(k[5]), _i(k[6]), _i(k[7]))
+
+
 class Solver:
     """One device-resident case: mesh, geometry, state Q and work arrays."""
 
@@ -127,13 +144,8 @@ class Solver:
     def __init__(self, mesh: Mesh, flow: Flow, precision: str = "fp64", ordering: str = "auto"):
         self.L = load()
         self.mesh, self.flow = mesh, flow
-        self._keep = [np.ascontiguousarray(a) for a in (
-            mesh.x_mm.astype(np.float64), mesh.y_mm.astype(np.float64), mesh.nod.astype(np.int32),
-            mesh.nc.astype(np.int32), mesh.parent.astype(np.int32), mesh.ghost.astype(np.int32),
-            mesh.btype.astype(np.int32), mesh.side.astype(np.int32))]
-        k = self._keep
-        m = _Mesh(mesh.nods, mesh.neles, mesh.nghosts, _d(k[0]), _d(k[1]), _i(k[2]), _i(k[3]),
-                  _i(k[4]), _i(k[5]), _i(k[6]), _i(k[7]))
+        self._keep = []
+        m = _mesh_struct(mesh, self._keep)
         p = make_params(flow)
         h = ctypes.c_void_p()
         rc = self.L.tdt_create(ctypes.byref(h), ctypes.byref(m), ctypes.byref(p),
@@ -268,6 +280,66 @@ class Solver:
 
     def synchronize(self):
         self._check(self.L.tdt_synchronize(self.h))
+
+
+class Batch:
+    """Independent cases advanced together (tdt_batch_*): one fused iteration over
+    the concatenated cells of every case, each case with its own flow constants.
+    Per case, the results are those of a Solver of that case alone."""
+
+    def __init__(self, cases, precision: str = "fp64", ordering: str = "auto"):
+        """cases: a sequence of (Mesh, Flow)."""
+        self.L = load()
+        self.cases = list(cases)
+        self._keep = []
+        meshes = (_Mesh * len(self.cases))(*[_mesh_struct(m, self._keep) for m, _ in self.cases])
+        params = (_Params * len(self.cases))(*[make_params(f) for _, f in self.cases])
+        h = ctypes.c_void_p()
+        rc = self.L.tdt_batch_create(ctypes.byref(h), len(self.cases), meshes, params,
+                                     PRECISION[precision], ORDERING[ordering])
+        if rc != TDT_OK:
+            raise TdtError(rc, self.L.tdt_last_error(None).decode())
+        self.h = h
+        self.ncases = len(self.cases)
+        self.neles = [m.neles for m, _ in self.cases]
+        self.precision = precision
+
+    close = Solver.close
+    __del__ = Solver.__del__
+    __enter__ = Solver.__enter__
+    __exit__ = Solver.__exit__
+    _check = Solver._check
+    _UNIFORM = Solver._UNIFORM
+    kernel_set = Solver.kernel_set
+    set_profiling = Solver.set_profiling
+    kernel_times = Solver.kernel_times
+    synchronize = Solver.synchronize
+
+    def cold_start(self) -> np.ndarray:
+        r = np.zeros(self.ncases)
+        self._check(self.L.tdt_batch_cold_start(self.h, _d(r)))
+        return r
+
+    def set_state(self, icase: int, c: np.ndarray, rinit: float) -> None:
+        c = np.ascontiguousarray(c, dtype=np.float64).reshape(5, self.neles[icase])
+        self._check(self.L.tdt_batch_set_state(self.h, int(icase), _d(c), float(rinit)))
+
+    def get_state(self, icase: int) -> np.ndarray:
+        out = np.empty((5, self.neles[icase]))
+        self._check(self.L.tdt_batch_get_state(self.h, int(icase), _d(out)))
+        return out
+
+    def iterate(self, niter: int, history: bool = True, check: bool = True):
+        """Returns (l2[niter, ncases, 5], linf[niter, ncases, 5], first_nonfinite)."""
+        l2 = np.zeros((niter, self.ncases, 5)) if history else None
+        linf = np.zeros((niter, self.ncases, 5)) if history else None
+        bad = ctypes.c_int32(0)
+        rc = self.L.tdt_batch_iterate(self.h, int(niter), _d(l2) if history else None,
+                                      _d(linf) if history else None, ctypes.byref(bad))
+        if rc == TDT_ERR_NONFINITE and not check:
+            return l2, linf, bad.value
+        self._check(rc)
+        return l2, linf, bad.value
 
 
 def version() -> str:

@@ -125,6 +125,32 @@ int tdt_postprocess(tdt_ctx *ctx, double *nodal);
 int tdt_iterate(tdt_ctx *ctx, int32_t niter, double *l2, double *linf,
                 int32_t *first_nonfinite);
 
+/* ---- batched cases ----------------------------------------------------------------
+ * No counterpart in the reference, which runs one case per process: for parameter
+ * sweeps, ncases independent grids advanced together.  One fused iteration (the
+ * kernels of tdt_iterate) runs over the concatenated cells of all cases; every
+ * cell block holds cells of one case and runs with that case's flow constants,
+ * so many small grids fill the GPU together.  Each case is renumbered as
+ * tdt_create(ordering) renumbers it alone, and its blocks are those of its own
+ * context: states and monitor norms are those of a single-case context, bit for
+ * bit in FP64.  The cases share LOCAL (= 1), IAXIS, GAMA, AMOL, CFL, PRANDTL,
+ * PRANDTLT and NPSEUDO; P0, T0, AMACHINIT, PAMB, RTRED and the type-7 inputs are
+ * per case; BC type 6 is not available (TDT_ERR_UNSUPPORTED).  The routine-level
+ * entry points, tdt_cold_start, tdt_set_state, tdt_get_state, tdt_iterate and
+ * tdt_postprocess refuse a batch context (TDT_ERR_UNSUPPORTED). */
+int tdt_batch_create(tdt_ctx **ctx, int32_t ncases, const tdt_mesh *meshes,
+                     const tdt_params *params, int32_t precision, int32_t ordering);
+/* Cold start of every case (turbnstwod.f90:53-71 with its own flow.in values);
+ * rinit: [ncases] or NULL. */
+int tdt_batch_cold_start(tdt_ctx *ctx, double *rinit);
+/* State of case icase: c = [5][neles of that case], its grid.in cell order. */
+int tdt_batch_set_state(tdt_ctx *ctx, int32_t icase, const double *c, double rinit);
+int tdt_batch_get_state(tdt_ctx *ctx, int32_t icase, double *c);
+/* niter iterations of every case; l2, linf: [niter][ncases][5] or NULL;
+ * first_nonfinite: the first iteration in which any case held NaN/Inf, or 0. */
+int tdt_batch_iterate(tdt_ctx *ctx, int32_t niter, double *l2, double *linf,
+                      int32_t *first_nonfinite);
+
 /* ---- arrays (reference names, original ids) --------------------------------------
  * Cell+ghost arrays have nmax entries (index = id-1; ids that are neither cell nor
  * ghost read 0): C1..C5 U V T P DT F1..F5 G1..G5 D2Q1..D2Q5 ANU ANURT XCC YCC VOL.

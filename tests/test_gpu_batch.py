@@ -1,0 +1,133 @@
+"""Batched cases on the GPU (tdt_batch_*): every case of a batch must come out
+exactly as it does in a context of its own.
+
+The batch renumbers each case as tdt_create would and gives it its own cell
+blocks, so in FP64 the states, the L2 and the Linf histories of each case are
+bitwise those of the single-case runs (same kernels, same blocks, same fold
+order).  The single-case runs are themselves pinned to the reference by
+test_gpu_fused.py."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ITERS = 30
+
+
+def small_sweep(n, seed=7):
+    """Sweep cases (both families, random Mach/P0/T0) on small grids."""
+    from tdt import gridgen
+    return [(m, f) for _, m, f in gridgen.sweep_cases(n, seed=seed, nx_range=(12, 60),
+                                                      ny_range=(8, 30))]
+
+
+def planar_cases(golden):
+    cases = small_sweep(5)
+    for name in ("t2d_chan", "t7_plate", "case1"):
+        g = golden(name)
+        cases.append((g.mesh, g.flow))
+    return cases
+
+
+def axi_cases(golden):
+    from tdt import gridgen
+    cases = [(golden(n).mesh, golden(n).flow) for n in ("taxi_duct", "case1_axi")]
+    cases.append((gridgen.duct(nx=40, ny=16, jitter=0.0), gridgen.flow_duct(amachinit=0.3)))
+    return cases
+
+
+def run_single(mesh, flow, precision, iters):
+    import tdt
+    with tdt.Solver(mesh, flow, precision) as s:
+        s.cold_start()
+        l2, linf, bad = s.iterate(iters)
+        assert bad == 0
+        return s.get_state(), l2, linf
+
+
+def check_bitwise(cases, precision, kernels, monkeypatch, iters=ITERS):
+    import tdt
+    monkeypatch.setenv("TDT_KERNELS", kernels)
+    with tdt.Batch(cases, precision) as b:
+        b.cold_start()
+        l2, linf, bad = b.iterate(iters)
+        assert bad == 0
+        assert b.kernel_set() == kernels
+        states = [b.get_state(k) for k in range(len(cases))]
+    for k, (mesh, flow) in enumerate(cases):
+        st, l2s, linfs = run_single(mesh, flow, precision, iters)
+        assert np.array_equal(states[k], st), \
+            f"case {k}: max diff {np.max(np.abs(states[k] - st))}"
+        assert np.array_equal(l2[:, k], l2s), f"case {k}: L2 history"
+        assert np.array_equal(linf[:, k], linfs), f"case {k}: Linf history"
+
+
+@pytest.mark.parametrize("kernels", ["merged", "split"])
+def test_batch_equals_single_cases_planar(golden, kernels, monkeypatch):
+    check_bitwise(planar_cases(golden), "fp64", kernels, monkeypatch)
+
+
+def test_batch_equals_single_cases_axisymmetric(golden, monkeypatch):
+    check_bitwise(axi_cases(golden), "fp64", "merged", monkeypatch)
+
+
+def test_batch_with_a_boundary_last_case(monkeypatch):
+    """A case of 263,680 cells is renumbered boundary-last inside the batch, as it
+    would be alone; small cases around it keep RCM."""
+    from tdt import gridgen
+    cases = small_sweep(2, seed=3)
+    cases.insert(1, (gridgen.ramp(nx=1030, ny=256), gridgen.flow_ramp()))
+    check_bitwise(cases, "fp64", "split", monkeypatch, iters=8)
+
+
+def test_batch_fp32_mixed_tracks_single_cases(golden, monkeypatch):
+    """FP32-mixed: the batch and single-case runs use kernels compiled apart, so
+    FMA contraction may differ; they agree to FP32 rounding."""
+    import tdt
+    monkeypatch.setenv("TDT_KERNELS", "merged")
+    cases = planar_cases(golden)
+    with tdt.Batch(cases, "fp32-mixed") as b:
+        b.cold_start()
+        b.iterate(ITERS)
+        states = [b.get_state(k) for k in range(len(cases))]
+    for k, (mesh, flow) in enumerate(cases):
+        st, _, _ = run_single(mesh, flow, "fp32-mixed", ITERS)
+        a = tdt.primitives_from_state(states[k], flow)
+        r = tdt.primitives_from_state(st, flow)
+        for q in ("rho", "p"):
+            err = np.linalg.norm(a[q] - r[q]) / np.linalg.norm(r[q])
+            assert err < 1e-6, (k, q, err)
+
+
+def test_batch_restart_and_state_round_trip(golden, monkeypatch):
+    """set_state / get_state per case: 20 + 20 iterations with the states written
+    back in between equal 40 straight iterations, bit for bit."""
+    import tdt
+    monkeypatch.setenv("TDT_KERNELS", "merged")
+    cases = planar_cases(golden)[:4]
+    with tdt.Batch(cases) as a, tdt.Batch(cases) as b:
+        ra = a.cold_start()
+        b.cold_start()
+        a.iterate(20)
+        mid = [a.get_state(k) for k in range(len(cases))]
+        for k in range(len(cases)):
+            a.set_state(k, mid[k], ra[k])
+            assert np.array_equal(a.get_state(k), mid[k])
+        a.iterate(20)
+        b.iterate(40)
+        for k in range(len(cases)):
+            assert np.array_equal(a.get_state(k), b.get_state(k)), k
+
+
+def test_batch_refuses_single_case_calls(golden):
+    import tdt
+    from tdt import lib
+    cases = planar_cases(golden)[:2]
+    with tdt.Batch(cases) as b:
+        b.cold_start()
+        for call in (lambda: b.L.tdt_timestep(b.h), lambda: b.L.tdt_iterate(b.h, 1, None, None, None),
+                     lambda: b.L.tdt_cold_start(b.h, None)):
+            assert call() == lib.TDT_ERR_UNSUPPORTED
+        buf = np.zeros(5 * max(b.neles))
+        assert b.L.tdt_batch_get_state(b.h, len(cases), lib._d(buf)) == lib.TDT_ERR_ARG
+        assert b.L.tdt_batch_get_state(b.h, -1, lib._d(buf)) == lib.TDT_ERR_ARG

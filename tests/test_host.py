@@ -143,3 +143,54 @@ def test_hicks_henne_bumps_and_adjoint_grid():
     x = flat.x_mm[top]
     want = np.array([gridgen.hicks_henne(v, amps) for v in x])
     assert np.allclose(bumped.y_mm[top] - flat.y_mm[top], want, atol=1e-9)
+
+
+def _sweep(n, seed=5):
+    return [(m, f) for _, m, f in gridgen.sweep_cases(n, seed=seed, nx_range=(12, 20),
+                                                      ny_range=(8, 12))]
+
+
+@pytest.mark.parametrize("mutate,code", [
+    (None, lib.TDT_ERR_DEVICE),                                         # valid: device check
+    (lambda c: dataclasses.replace(c, local=0), lib.TDT_ERR_UNSUPPORTED),
+    (lambda c: dataclasses.replace(c, iaxis=1), lib.TDT_ERR_UNSUPPORTED),
+    (lambda c: dataclasses.replace(c, gama=1.3), lib.TDT_ERR_UNSUPPORTED),
+    (lambda c: dataclasses.replace(c, cfl=0.0), lib.TDT_ERR_ARG),
+])
+def test_batch_validation(mutate, code):
+    """A batch checks every case as tdt_create would, and that the cases share
+    the gas and model constants, before it looks for a device."""
+    cases = _sweep(3)
+    if mutate:
+        m, f = cases[1]
+        cases[1] = (m, mutate(f))
+    with pytest.raises(tdt.TdtError) as ei:
+        tdt.Batch(cases)
+    assert ei.value.code == code, str(ei.value)
+
+
+def test_batch_names_the_bad_case():
+    cases = _sweep(3)
+    m, f = cases[2]
+    m = caseio.Mesh.from_arrays({k: v.copy() for k, v in m.arrays().items()})
+    m.nod[0, 0] = 0
+    cases[2] = (m, f)
+    with pytest.raises(tdt.TdtError) as ei:
+        tdt.Batch(cases)
+    assert ei.value.code == lib.TDT_ERR_MESH and "case 2" in str(ei.value)
+
+
+def test_sweep_cases_are_reproducible_and_in_range():
+    a = gridgen.sweep_cases(6, seed=11, nx_range=(65, 129), ny_range=(33, 65))
+    b = gridgen.sweep_cases(6, seed=11, nx_range=(65, 129), ny_range=(33, 65))
+    for (fa, ma, qa), (fb, mb, qb) in zip(a, b):
+        assert fa == fb and qa == qb
+        for k, v in ma.arrays().items():
+            assert np.array_equal(v, getattr(mb, k)), k
+    for fam, m, f in a:
+        assert fam in ("channel", "ramp")
+        assert 0.2 <= f.amachinit <= 2.0 and 1e5 <= f.p0 <= 2e6 and 250.0 <= f.t0 <= 400.0
+        assert f.cfl == 0.25 and f.niter == 5000 and f.local == 1
+        assert 65 * 33 <= m.neles <= 129 * 65
+        if fam == "channel":
+            assert f.deltabl > 0 and f.aminf == f.amachinit
