@@ -241,6 +241,56 @@ def large_case(stream_gbs, steps=200, warmup=50):
     return out
 
 
+def batch_sweep(ncases=64, steps=100, warmup=20, seq_steps=50, precision="fp32-mixed"):
+    """The batched parameter sweep (gridgen.sweep_cases: 64 cases of both
+    families, uneven sizes): all cases in one batch context, one fused iteration
+    per step over the concatenated cells, against the same cases run one after
+    another in contexts of their own (each timed alone over seq_steps; the
+    sequential rate is total cell-iterations over the summed times)."""
+    cases = gridgen.sweep_cases(ncases)
+    sizes = [m.neles for _, m, _ in cases]
+    ntot = sum(sizes)
+    pairs = [(m, f) for _, m, f in cases]
+    with tdt.Batch(pairs, precision) as b:
+        b.cold_start()
+        b.iterate(warmup, history=False, check=False)
+        b.synchronize()
+        t0 = time.perf_counter()
+        _, _, bad = b.iterate(steps, history=False, check=False)
+        b.synchronize()
+        el = time.perf_counter() - t0
+        ks = b.kernel_set()
+    batched = ntot * steps / el / 1e6
+    print(f"bench: batch of {ncases} ({ntot} cells) {precision} {batched:.1f} Mcell-iters/s "
+          f"kernel set {ks}", file=sys.stderr, flush=True)
+    seq_time = 0.0
+    seq_bad = 0
+    for m, f in pairs:
+        with tdt.Solver(m, f, precision) as s:
+            s.cold_start()
+            s.iterate(warmup, history=False, check=False)
+            s.synchronize()
+            t0 = time.perf_counter()
+            _, _, bad1 = s.iterate(seq_steps, history=False, check=False)
+            s.synchronize()
+            seq_time += (time.perf_counter() - t0) * steps / seq_steps
+            seq_bad = seq_bad or bad1
+    sequential = ntot * steps / seq_time / 1e6
+    print(f"bench: the same cases one after another {sequential:.1f} Mcell-iters/s",
+          file=sys.stderr, flush=True)
+    fam = [c[0] for c in cases]
+    return {"workload": f"sweep of {ncases} cases ({fam.count('channel')} channel, "
+                        f"{fam.count('ramp')} ramp), {ntot} cells, largest {max(sizes)}, "
+                        f"smallest {min(sizes)}",
+            "precision": precision, "steps": steps, "warmup": warmup,
+            "batched": {"value": round(batched, 2), "unit": "Mcell-iters/s",
+                        "ms_per_step": round(el * 1e3 / steps, 4), "kernel_set": ks,
+                        "nonfinite_iter": int(bad)},
+            "sequential": {"value": round(sequential, 2), "unit": "Mcell-iters/s",
+                           "timed_steps_per_case": seq_steps, "nonfinite_iter": int(seq_bad)},
+            "speedup": round(batched / sequential, 3)}
+
+
 def copy_probe(mib: int) -> None:
     """A plain elementwise copy of known size: y = x * 1 over MIB MiB of float64, three
     times.  Its FETCH_SIZE / WRITE_SIZE calibrate the HBM counters (expect MIB MiB each)."""
@@ -266,6 +316,9 @@ def main():
     ap.add_argument("--stream-probe", type=int, default=0, metavar="MIB",
                     help="print the bandwidth of a plain MIB MiB device copy (GB/s) and exit")
     ap.add_argument("--stream-reps", type=int, default=20, help=argparse.SUPPRESS)
+    ap.add_argument("--batch", type=int, default=0, metavar="NCASES",
+                    help="measure the batched parameter sweep of NCASES cases against the same "
+                         "cases run one after another, print its JSON line and exit")
     ap.add_argument("--no-large", action="store_true",
                     help="skip the 2049x513 ramp roofline run beside the main workload")
     ap.add_argument("--converge", type=int, default=-1, metavar="ITERS",
@@ -291,6 +344,12 @@ def main():
                                         args.ordering)
         print(f"bench: {args.precision} {mesh.neles * args.steps / el / 1e6:.1f} Mcell-iters/s "
               f"kernel set {ks} kernels(ms) {np.round(kt, 5).tolist()}", flush=True)
+        return
+    if args.batch:
+        out = batch_sweep(args.batch, steps=args.steps if args.steps != 2000 else 100,
+                          warmup=min(args.warmup, 20))
+        print(json.dumps({"metric": "batched sweep, aggregate Mcell-iters/s on 1 MI355X",
+                          **out}), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); multi-GPU "
