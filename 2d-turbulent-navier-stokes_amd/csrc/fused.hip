@@ -155,12 +155,12 @@ int fused_setup(tdt_ctx *c) {
         B->tuned = true;
     } else if (!ks.empty() && ks.find_first_not_of("0123456789") == std::string::npos) {
         const int v = std::atoi(ks.c_str());
-        if ((v & ~(kPreQuad | kSt1Mask | kStQuad)) == 0 && (v & kSt1Mask) != kSt1Mask) {
+        if ((v & ~kSetBits) == 0 && (v & kSt1Mask) != kSt1Mask) {
             B->kset = v;
             B->tuned = true;
         }
     }
-    if (c->batch) B->kset &= kSt1Split;   // no quad form for a batch
+    if (c->batch) B->kset &= kSt1Split | kStPreload;   // no quad form for a batch
     // fused-only neighbour and node-gather tables (built from the prepared mesh)
     const PreparedMesh &pm = c->pm;
     if (D.nt > kIdMask) {
@@ -334,6 +334,16 @@ static hipError_t autotune(tdt_ctx *c, FusedBufs &B) {
         if (e == hipSuccess)
             e = hipMemcpyAsync(F.qn, B.qbak, qbytes, hipMemcpyDeviceToDevice, c->stream);
         if (e == hipSuccess && ms < best_ms) { best_ms = ms; best = cand[k]; }
+    }
+    // FP64: the one-lane stage kernels with neighbour states preloaded, if the
+    // chosen set has any
+    if (e == hipSuccess && c->precision == TDT_FP64 &&
+        ((best & kSt1Mask) == kSt1Split || !(best & kStQuad))) {
+        float ms = 0.f;
+        e = time_set(c, B, best | kStPreload, kTuneIterations, nullptr, &ms);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(F.qn, B.qbak, qbytes, hipMemcpyDeviceToDevice, c->stream);
+        if (e == hipSuccess && ms < best_ms) best = best | kStPreload;
     }
     c->profiling = prof;
     B.kset = best;

@@ -86,12 +86,19 @@ struct FusedBufs {
 };
 
 // Kernel set of one iteration, chosen per slot; every choice computes the same
-// values (FP64: bitwise).  Bit 0: pre-pass; bits 1-2: stage 1; bit 3: stages 2-4.
+// values (FP64: bitwise).  Bit 0: pre-pass; bits 1-2: stage 1; bit 3: stages 2-4;
+// bit 4: how the one-lane stage kernels load neighbour states.
 constexpr int kPreQuad = 1;        // kq_pre instead of k_pre
 constexpr int kSt1Split = 1 << 1;  // k_dv + k_rk<1, DV=false> instead of k_rk<1> (merged)
 constexpr int kSt1Quad = 2 << 1;   // kq_stage<1>
 constexpr int kSt1Mask = 3 << 1;
 constexpr int kStQuad = 1 << 3;    // kq_stage<2..4> instead of k_rk<2..4>
+// FP64 one-lane k_rk (split stage 1, stages 2-4): all four neighbours' states
+// requested up front instead of face by face.  More registers (one wavefront
+// per SIMD), one memory round trip instead of four: pays when the grid is too
+// small to give a SIMD more than one wavefront anyway.  FP32 always preloads.
+constexpr int kStPreload = 1 << 4;
+constexpr int kSetBits = kPreQuad | kSt1Mask | kStQuad | kStPreload;
 // the three uniform sets
 constexpr int kSetMerged = 0;
 constexpr int kSetSplit = kSt1Split;

@@ -485,7 +485,7 @@ __global__ void __launch_bounds__(kBlock) k_dv(FDev F, Consts<double> K, Consts<
 // its share of the source-term gradient differences.  SI / SO: stage input /
 // output storage types; SD: storage of D, VIS, sensors and nodal averages.
 // DV = false: stage 1 reads D and VIS written by k_dv (the split variant).
-template <class R, class SI, class SO, class SD, int STAGE, bool DV, bool B6, bool BT>
+template <class R, class SI, class SO, class SD, int STAGE, bool DV, bool B6, bool BT, bool PL>
 __global__ void __launch_bounds__(kBlock) k_rk(FDev F, Consts<double> K, Consts<R> KR,
                                                double alpha) {
     constexpr bool kInlineDV = STAGE == 1 && DV;
@@ -561,9 +561,10 @@ __global__ void __launch_bounds__(kBlock) k_rk(FDev F, Consts<double> K, Consts<
 
         // FP32: every face's neighbour state is loaded up front (a ghost face
         // loads this cell again), so no load waits on a branch.  FP64 loads
-        // them face by face: holding all twenty would cost the second
-        // wavefront per SIMD (256 VGPRs).
-        constexpr bool kPreload = sizeof(R) == 4;
+        // them face by face unless PL: holding all twenty costs the second
+        // wavefront per SIMD (256 VGPRs), which only grids too small to give a
+        // SIMD two wavefronts can spare.
+        constexpr bool kPreload = PL || sizeof(R) == 4;
         R qf[4][5];
 #pragma unroll
         for (int f = 0; f < 4; ++f) {
@@ -730,10 +731,27 @@ void launch(tdt_ctx *c, FusedBufs &B, int slot, K kern, unsigned grid, Args... a
         hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, c->stream, args...);
 }
 
+// One-lane stage kernel, with the neighbour states preloaded or not (FP64; FP32
+// always preloads).
+template <class R, class SI, class SO, class SD, int STAGE, bool DV, bool B6, bool BT>
+void launch_rk(tdt_ctx *c, FusedBufs &B, int slot, bool preload, const Consts<R> &KR,
+               double alpha) {
+    if constexpr (sizeof(R) == 8) {
+        if (preload) {
+            launch(c, B, slot, k_rk<R, SI, SO, SD, STAGE, DV, B6, BT, true>, B.F.ncb8, B.F,
+                   c->K, KR, alpha);
+            return;
+        }
+    }
+    launch(c, B, slot, k_rk<R, SI, SO, SD, STAGE, DV, B6, BT, false>, B.F.ncb8, B.F, c->K, KR,
+           alpha);
+}
+
 template <class R, class SS, class SD, bool B6, bool BT>
 hipError_t enqueue_iteration(tdt_ctx *c, FusedBufs &B, const Consts<R> &KR) {
     const FDev &F = B.F;
-    const int ks = BT ? B.kset & kSt1Split : B.kset;   // a batch has no quad form
+    const int ks = BT ? B.kset & (kSt1Split | kStPreload) : B.kset;   // a batch: no quad form
+    const bool pl = ks & kStPreload;
     const unsigned nbc = F.ncb8, nq = B.nbq_cells;
     const int nparts = partial_blocks(B, ks);
     auto no_dv_slot = [&]() {   // slot 1 unused: dissipation and viscous live in stage 1
@@ -760,7 +778,7 @@ hipError_t enqueue_iteration(tdt_ctx *c, FusedBufs &B, const Consts<R> &KR) {
     switch (ks & kSt1Mask) {
     case kSt1Split:
         launch(c, B, 1, k_dv<SD, R, B6, BT>, nbc, F, c->K, KR);
-        launch(c, B, 2, k_rk<R, double, SS, SD, 1, false, B6, BT>, nbc, F, c->K, KR, 0.25);
+        launch_rk<R, double, SS, SD, 1, false, B6, BT>(c, B, 2, pl, KR, 0.25);
         break;
     case kSt1Quad:
         no_dv_slot();
@@ -768,7 +786,7 @@ hipError_t enqueue_iteration(tdt_ctx *c, FusedBufs &B, const Consts<R> &KR) {
         break;
     default:
         no_dv_slot();
-        launch(c, B, 2, k_rk<R, double, SS, SD, 1, true, B6, BT>, nbc, F, c->K, KR, 0.25);
+        launch(c, B, 2, k_rk<R, double, SS, SD, 1, true, B6, BT, false>, nbc, F, c->K, KR, 0.25);
     }
     if (ks & kStQuad) {
         bc6(k_bc6<R, SS, SD, 2>);
@@ -779,11 +797,11 @@ hipError_t enqueue_iteration(tdt_ctx *c, FusedBufs &B, const Consts<R> &KR) {
         launch(c, B, 5, kq_stage<R, SS, double, SD, 4, B6>, nq, F, c->K, KR, 1.0);
     } else {
         bc6(k_bc6<R, SS, SD, 2>);
-        launch(c, B, 3, k_rk<R, SS, SS, SD, 2, true, B6, BT>, nbc, F, c->K, KR, 1.0 / 3.0);
+        launch_rk<R, SS, SS, SD, 2, true, B6, BT>(c, B, 3, pl, KR, 1.0 / 3.0);
         bc6(k_bc6<R, SS, SD, 3>);
-        launch(c, B, 4, k_rk<R, SS, SS, SD, 3, true, B6, BT>, nbc, F, c->K, KR, 0.5);
+        launch_rk<R, SS, SS, SD, 3, true, B6, BT>(c, B, 4, pl, KR, 0.5);
         bc6(k_bc6<R, SS, SD, 4>);
-        launch(c, B, 5, k_rk<R, SS, double, SD, 4, true, B6, BT>, nbc, F, c->K, KR, 1.0);
+        launch_rk<R, SS, double, SD, 4, true, B6, BT>(c, B, 5, pl, KR, 1.0);
     }
     return hipGetLastError();
 }

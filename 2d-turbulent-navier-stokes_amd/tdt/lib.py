@@ -265,18 +265,21 @@ class Solver:
 
     def kernel_set(self) -> str:
         """The fused kernel set in use: "merged", "split", "quad", or for a mixed
-        choice e.g. "pre=quad,stage1=merged,stages=lane" ("" before the first
-        tdt_iterate)."""
+        choice e.g. "pre=quad,stage1=merged,stages=lane", with "+preload" when the
+        FP64 one-lane stage kernels load all neighbour states up front ("" before
+        the first tdt_iterate)."""
         out = np.zeros(1, dtype=np.int32)
         self._check(self.L.tdt_kernel_set(self.h, _i(out)))
         v = int(out[0])
         if v < 0:
             return ""
+        pl = "+preload" if v & 16 else ""     # FP64 one-lane stages preload neighbours
+        v &= 15
         if v in self._UNIFORM:
-            return self._UNIFORM[v]
+            return self._UNIFORM[v] + pl
         st1 = {0: "merged", 2: "split", 4: "quad"}[v & 6]
         return (f"pre={'quad' if v & 1 else 'lane'},stage1={st1},"
-                f"stages={'quad' if v & 8 else 'lane'}")
+                f"stages={'quad' if v & 8 else 'lane'}{pl}")
 
     def synchronize(self):
         self._check(self.L.tdt_synchronize(self.h))

@@ -181,6 +181,10 @@ int tdt_kernel_times(tdt_ctx *ctx, double *times_ms);
 #define TDT_KERNELS_STAGE1_SPLIT 2    /* dissipation + viscous in a kernel of their own */
 #define TDT_KERNELS_STAGE1_QUAD 4     /* stage 1 (with D and VIS): four lanes per cell */
 #define TDT_KERNELS_STAGES_QUAD 8     /* stages 2-4: four lanes per cell               */
+#define TDT_KERNELS_PRELOAD 16        /* FP64 one-lane stage kernels (split stage 1,
+                                         stages 2-4) request all neighbour states up
+                                         front: one memory round trip instead of four,
+                                         at one wavefront per SIMD                     */
 #define TDT_KERNELS_MERGED 0          /* uniform sets: one lane per cell, D/VIS in stage 1 */
 #define TDT_KERNELS_SPLIT 2
 #define TDT_KERNELS_QUAD 13

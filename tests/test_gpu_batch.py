@@ -45,6 +45,9 @@ def run_single(mesh, flow, precision, iters):
         return s.get_state(), l2, linf
 
 
+SET_NAMES = {"merged": "merged", "split": "split", "18": "split+preload"}
+
+
 def check_bitwise(cases, precision, kernels, monkeypatch, iters=ITERS):
     import tdt
     monkeypatch.setenv("TDT_KERNELS", kernels)
@@ -52,7 +55,7 @@ def check_bitwise(cases, precision, kernels, monkeypatch, iters=ITERS):
         b.cold_start()
         l2, linf, bad = b.iterate(iters)
         assert bad == 0
-        assert b.kernel_set() == kernels
+        assert b.kernel_set() == SET_NAMES[kernels]
         states = [b.get_state(k) for k in range(len(cases))]
     for k, (mesh, flow) in enumerate(cases):
         st, l2s, linfs = run_single(mesh, flow, precision, iters)
@@ -62,7 +65,7 @@ def check_bitwise(cases, precision, kernels, monkeypatch, iters=ITERS):
         assert np.array_equal(linf[:, k], linfs), f"case {k}: Linf history"
 
 
-@pytest.mark.parametrize("kernels", ["merged", "split"])
+@pytest.mark.parametrize("kernels", ["merged", "split", "18"])
 def test_batch_equals_single_cases_planar(golden, kernels, monkeypatch):
     check_bitwise(planar_cases(golden), "fp64", kernels, monkeypatch)
 

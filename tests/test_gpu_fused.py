@@ -180,8 +180,9 @@ def test_kernel_sets_identical(golden, case, precision, monkeypatch):
     import tdt
     g = golden(case)
     runs = {}
-    # the uniform sets and mixed choices (pre-pass quad; stage 1 quad; stages 2-4 quad)
-    for ks in tdt.Solver.KERNEL_SETS + ("1", "4", "8", "5"):
+    # the uniform sets and mixed choices (pre-pass quad; stage 1 quad; stages 2-4 quad;
+    # merged and split with the FP64 one-lane stages preloading their neighbours)
+    for ks in tdt.Solver.KERNEL_SETS + ("1", "4", "8", "5", "16", "18"):
         monkeypatch.setenv("TDT_KERNELS", ks)
         with tdt.Solver(g.mesh, g.flow, precision) as s:
             s.cold_start()
