@@ -1,0 +1,238 @@
+// Adjoint of the residual: g = v^T dR/dQ and v^T dR/d(P0, T0) for the R1..R5 of
+// COMPUTE_RESIDUAL (residual.f90:23-62) evaluated after COMPUTE_PRIMITIVE_VARS
+// (turbnstwod.f90:204-227), APPLY_BOUNDARY (boundary.f90:24-122, 310-327) and
+// COMPUTE_SOURCE_TERMS (source_terms.f90:24-104) on a state Q.
+//
+// R_i depends on the states of cell i and of its face neighbours only (a ghost
+// is built from its parent, which is i, and for type 3 from the neighbour
+// across i's opposite face, itself one of i's neighbours).  So dR/dQ is a
+// block-sparse matrix with one 5x5 block per (cell, stencil member):
+//   k_res_jac  forms, per cell, the five blocks dR_i/dQ_s (s = i, then the
+//              neighbours across faces 1..4) and dR_i/d(P0, T0), by forward-
+//              mode dual numbers through the same physics functions the solver
+//              runs (phys.hpp), so the value part is the solver's R bit for bit;
+//   k_res_vjp  applies the transpose as a gather: g_k = sum over the (i, s)
+//              whose member s is k, in increasing i, of v_i^T dR_i/dQ_s -- the
+//              transposed stencil, deterministic, no atomics;
+//   k_par_vjp  + k_par_fold: v^T dR/d(P0, T0) as block partial sums folded in a
+//              fixed order.
+#include "ctx.hpp"
+#include "dual.hpp"
+#include "reduce.hpp"
+
+namespace tdt {
+namespace {
+
+constexpr int kJacBlocks = 5;         // stencil members per cell: self, faces 1..4
+constexpr int kJacStride = 5 * 5 * kJacBlocks;
+constexpr int kDualN = 5;             // directions per dual evaluation
+
+__device__ __forceinline__ int gid() { return blockIdx.x * kBlock + threadIdx.x; }
+
+// R_i from the states q[0] (cell i) and q[1 + f] (neighbour across face f+1;
+// unused for a ghost face), with the same operations in the same order as
+// k_primitives, k_bc, k_source and k_residual.
+template <class R>
+__device__ void cell_residual(const Dev &D, int i, const R (&q)[5][5], const Consts<R> &K,
+                              R (&r)[5]) {
+    const int n = D.n;
+    const int4 nb4 = D.nc4[i];
+    const int nb[4] = {nb4.x, nb4.y, nb4.z, nb4.w};
+    Prim<R> pi, pj[4];
+    prim_from_q<R>(pi, q[0][0], q[0][1], q[0][2], q[0][3], q[0][4], K);
+    for (int f = 0; f < 4; ++f)
+        if (nb[f] < n) prim_from_q<R>(pj[f], q[1 + f][0], q[1 + f][1], q[1 + f][2], q[1 + f][3],
+                                      q[1 + f][4], K);
+    for (int f = 0; f < 4; ++f) {
+        if (nb[f] < n) continue;
+        const int k = nb[f] - n, o = (f + 2) & 3;
+        const int ty = D.gtype[k];
+        R uo = R(0), vo = R(0), to = R(0), po = R(0);
+        if (ty == 3) {   // the parent's opposite neighbour: a cell (checked at set-up)
+            uo = pj[o].u; vo = pj[o].v; to = pj[o].t; po = pj[o].p;
+        }
+        ghost_state<R>(pj[f], ty, pi, uo, vo, to, po, R(D.ycc[nb[f]]), K);
+    }
+    // COMPUTE_SOURCE_TERMS
+    R s3;
+    auto nbr = [&](int f) {
+        return src_nbr<R>(R(D.xcc[nb[f]]), R(D.ycc[nb[f]]), pj[f].u, pj[f].v, pj[f].c[0],
+                          pj[f].c[4]);
+    };
+    const R s5 = source_terms<R>(nbr(0), nbr(1), nbr(2), nbr(3), pi.c[0], pi.c[4], pi.t, pi.p,
+                                 pi.v, R(D.ycc[i]), s3, K);
+    // COMPUTE_RESIDUAL
+    R sc[8];
+    for (int k = 0; k < 8; ++k) sc[k] = R(D.sc[k * n + i]);
+    for (int m = 0; m < 5; ++m) {
+        R s = (pi.f[m] + pj[0].f[m]) * sc[0] + (pi.g[m] + pj[0].g[m]) * sc[1];
+        for (int k = 1; k < 4; ++k) {
+            s = s + (pi.f[m] + pj[k].f[m]) * sc[2 * k];
+            s = s + (pi.g[m] + pj[k].g[m]) * sc[2 * k + 1];
+        }
+        r[m] = R(0.5) * s;
+    }
+    r[4] = r[4] - R(D.vol[i]) * s5;
+    if (K.iaxis == 1) r[2] = r[2] - R(D.vol[i]) * s3;
+}
+
+// The stencil states of cell i from qs ([5][n], internal order): q[0] = cell,
+// q[1 + f] = face neighbour (the cell itself on a ghost face, unused there).
+__device__ __forceinline__ void load_stencil(const Dev &D, const double *qs, int i,
+                                             double (&q)[5][5]) {
+    const int n = D.n;
+    const int4 nb4 = D.nc4[i];
+    const int nb[4] = {nb4.x, nb4.y, nb4.z, nb4.w};
+    for (int s = 0; s < 5; ++s) {
+        const int j = s == 0 ? i : (nb[s - 1] < n ? nb[s - 1] : i);
+        for (int c = 0; c < 5; ++c) q[s][c] = qs[(size_t)c * n + j];
+    }
+}
+
+// r = R(Q) in FP64 (the forward evaluation behind the adjoint).
+__global__ void __launch_bounds__(kBlock) k_res_eval(Dev D, Consts<double> K, const double *qs,
+                                                     double *r) {
+    const int i = gid();
+    if (i >= D.n) return;
+    double q[5][5], ri[5];
+    load_stencil(D, qs, i, q);
+    cell_residual<double>(D, i, q, K, ri);
+    for (int m = 0; m < 5; ++m) r[(size_t)m * D.n + i] = ri[m];
+}
+
+// Derivatives of the flow constants with respect to P0 (direction 0) and T0
+// (direction 1): S0 = P0^(1-GAMA) (RR T0)^GAMA, T0, TINF = T0 / (1 + (GAMA-1)/2
+// AMINF^2), and BL_U = 2 AMINF sqrt(GAMA RR TINF).
+struct ParamSeeds {
+    double ds0_dp0, ds0_dt0, dtinf_dt0, dblu_dt0;
+};
+
+// J[i] = [s][m][c]: dR_i,m / dQ_s,c (zero blocks for ghost faces); Jp[i] =
+// [m][p]: dR_i,m / d(P0, T0).
+__global__ void __launch_bounds__(kBlock) k_res_jac(Dev D, Consts<double> K, ParamSeeds ps,
+                                                    const double *qs, double *J, double *Jp) {
+    const int i = gid();
+    if (i >= D.n) return;
+    const int n = D.n;
+    const int4 nb4 = D.nc4[i];
+    const int nb[4] = {nb4.x, nb4.y, nb4.z, nb4.w};
+    double qd[5][5];
+    load_stencil(D, qs, i, qd);
+    const Consts<Dual<kDualN>> KD = lift<kDualN>(K);
+    double *Ji = J + (size_t)i * kJacStride;
+    for (int s = 0; s < kJacBlocks; ++s) {
+        if (s > 0 && nb[s - 1] >= n) {
+            for (int e = 0; e < 25; ++e) Ji[s * 25 + e] = 0.0;
+            continue;
+        }
+        Dual<kDualN> q[5][5], r[5];
+        for (int a = 0; a < 5; ++a)
+            for (int c = 0; c < 5; ++c) q[a][c] = Dual<kDualN>(qd[a][c]);
+        for (int c = 0; c < 5; ++c) q[s][c].d[c] = 1.0;   // direction c: Q_s,c
+        cell_residual<Dual<kDualN>>(D, i, q, KD, r);
+        for (int m = 0; m < 5; ++m)
+            for (int c = 0; c < 5; ++c) Ji[s * 25 + m * 5 + c] = r[m].d[c];
+    }
+    // parameters: directions 0 (P0) and 1 (T0) of the constants that hold them
+    Consts<Dual<kDualN>> KP = KD;
+    KP.s0.d[0] = ps.ds0_dp0;
+    KP.s0.d[1] = ps.ds0_dt0;
+    KP.t0.d[1] = 1.0;
+    KP.tinf.d[1] = ps.dtinf_dt0;
+    KP.bl_u.d[1] = ps.dblu_dt0;
+    Dual<kDualN> q[5][5], r[5];
+    for (int a = 0; a < 5; ++a)
+        for (int c = 0; c < 5; ++c) q[a][c] = Dual<kDualN>(qd[a][c]);
+    cell_residual<Dual<kDualN>>(D, i, q, KP, r);
+    for (int m = 0; m < 5; ++m) {
+        Jp[(size_t)i * 10 + m * 2] = r[m].d[0];
+        Jp[(size_t)i * 10 + m * 2 + 1] = r[m].d[1];
+    }
+}
+
+// g_k = sum over the entries (i, s) of cell k's transposed stencil, in order,
+// of v_i^T dR_i/dQ_s.
+__global__ void __launch_bounds__(kBlock) k_res_vjp(Dev D, const double *J, const double *v,
+                                                    const int *tptr, const int2 *tlist,
+                                                    double *g) {
+    const int k = gid();
+    if (k >= D.n) return;
+    const int n = D.n;
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int e = tptr[k]; e < tptr[k + 1]; ++e) {
+        const int2 is = tlist[e];
+        const double *B = J + (size_t)is.x * kJacStride + is.y * 25;
+        double vi[5];
+        for (int m = 0; m < 5; ++m) vi[m] = v[(size_t)m * n + is.x];
+        for (int c = 0; c < 5; ++c) {
+            double t = vi[0] * B[c];
+            for (int m = 1; m < 5; ++m) t = t + vi[m] * B[m * 5 + c];
+            acc[c] = acc[c] + t;
+        }
+    }
+    for (int c = 0; c < 5; ++c) g[(size_t)c * n + k] = acc[c];
+}
+
+// per block: the sums over its cells of v_i^T dR_i/d(P0, T0)
+__global__ void __launch_bounds__(kBlock) k_par_vjp(Dev D, const double *Jp, const double *v,
+                                                    double *part) {
+    const int i = gid();
+    double t[2] = {0.0, 0.0};
+    if (i < D.n)
+        for (int p = 0; p < 2; ++p) {
+            double a = v[i] * Jp[(size_t)i * 10 + p];
+            for (int m = 1; m < 5; ++m) a = a + v[(size_t)m * D.n + i] * Jp[(size_t)i * 10 + m * 2 + p];
+            t[p] = a;
+        }
+    __shared__ double sh[kBlock / kWave][2];
+    const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+    for (int p = 0; p < 2; ++p) {
+        const double s = wave_sum(t[p]);
+        if (lane == 0) sh[w][p] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        double a = sh[0][threadIdx.x];
+        for (int q = 1; q < kBlock / kWave; ++q) a = a + sh[q][threadIdx.x];
+        part[blockIdx.x * 2 + threadIdx.x] = a;
+    }
+}
+
+// one thread per parameter: the block partials in block order
+__global__ void k_par_fold(const double *part, int nblk, double *out) {
+    const int p = threadIdx.x;
+    if (p >= 2) return;
+    double a = 0.0;
+    for (int b = 0; b < nblk; ++b) a = a + part[b * 2 + p];
+    out[p] = a;
+}
+
+}  // namespace
+
+hipError_t launch_res_eval(tdt_ctx *c, const double *qs, double *r) {
+    hipLaunchKernelGGL(k_res_eval, dim3(nblocks(c->D.n)), dim3(kBlock), 0, c->stream, c->D, c->K,
+                       qs, r);
+    return hipGetLastError();
+}
+
+hipError_t launch_res_vjp(tdt_ctx *c, const double *qs, const double *v, const int *tptr,
+                          const int2 *tlist, double *J, double *Jp, double *part, double *g,
+                          double *gp) {
+    const Consts<double> &K = c->K;
+    const tdt_params &P = c->params;
+    ParamSeeds ps;
+    const double gam = P.gama;
+    ps.ds0_dp0 = (1.0 - gam) * (double)K.s0 / P.p0;
+    ps.ds0_dt0 = gam * (double)K.s0 / P.t0;
+    ps.dtinf_dt0 = (double)K.tinf / P.t0;
+    ps.dblu_dt0 = 0.5 * (double)K.bl_u / P.t0;
+    const unsigned nb = nblocks(c->D.n);
+    hipLaunchKernelGGL(k_res_jac, dim3(nb), dim3(kBlock), 0, c->stream, c->D, K, ps, qs, J, Jp);
+    hipLaunchKernelGGL(k_res_vjp, dim3(nb), dim3(kBlock), 0, c->stream, c->D, J, v, tptr, tlist,
+                       g);
+    hipLaunchKernelGGL(k_par_vjp, dim3(nb), dim3(kBlock), 0, c->stream, c->D, Jp, v, part);
+    hipLaunchKernelGGL(k_par_fold, dim3(1), dim3(kWave), 0, c->stream, part, (int)nb, gp);
+    return hipGetLastError();
+}
+
+}  // namespace tdt

@@ -313,6 +313,78 @@ int batch_case_range(tdt_ctx *c, int icase, int *first, int *count) {
     return TDT_OK;
 }
 
+int adj_setup(tdt_ctx *c) {
+    if (c->adj) return TDT_OK;
+    if (!c->pm.g6cell.empty()) {
+        c->err = "residual adjoint: BC type 6 is not available";
+        return TDT_ERR_UNSUPPORTED;
+    }
+    const tdt::PreparedMesh &pm = c->pm;
+    const int n = pm.n;
+    // transposed stencil: entries (i, s) with member s of cell i equal to k, by i
+    std::vector<int> ptr(n + 1, 0);
+    auto member = [&](int i, int s) { return s == 0 ? i : pm.nc4[(size_t)4 * i + s - 1]; };
+    for (int i = 0; i < n; ++i)
+        for (int s = 0; s < 5; ++s) {
+            const int k = member(i, s);
+            if (k < n) ++ptr[k + 1];
+        }
+    for (int k = 0; k < n; ++k) ptr[k + 1] += ptr[k];
+    std::vector<int2> list(ptr[n]);
+    std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+    for (int i = 0; i < n; ++i)
+        for (int s = 0; s < 5; ++s) {
+            const int k = member(i, s);
+            if (k < n) list[fill[k]++] = make_int2(i, s);
+        }
+    auto a = std::make_unique<tdt::AdjBufs>();
+    a->q = dev_alloc<double>(c, (size_t)5 * n);
+    a->v = dev_alloc<double>(c, (size_t)5 * n);
+    a->g = dev_alloc<double>(c, (size_t)5 * n);
+    a->r = dev_alloc<double>(c, (size_t)5 * n);
+    a->J = dev_alloc<double>(c, (size_t)125 * n, false);
+    a->Jp = dev_alloc<double>(c, (size_t)10 * n, false);
+    a->part = dev_alloc<double>(c, (size_t)2 * tdt::nblocks(n));
+    a->gp = dev_alloc<double>(c, 2);
+    a->tptr = dev_alloc<int>(c, n + 1);
+    a->tlist = dev_alloc<int2>(c, list.size());
+    bool ok = a->q && a->v && a->g && a->r && a->J && a->Jp && a->part && a->gp && a->tptr &&
+              a->tlist;
+    ok = ok && hipMemcpy(a->tptr, ptr.data(), ptr.size() * sizeof(int),
+                         hipMemcpyHostToDevice) == hipSuccess;
+    ok = ok && hipMemcpy(a->tlist, list.data(), list.size() * sizeof(int2),
+                         hipMemcpyHostToDevice) == hipSuccess;
+    if (!ok) {
+        c->err = "residual adjoint: device allocation failed";
+        return TDT_ERR_DEVICE;
+    }
+    c->adj = a.release();
+    return TDT_OK;
+}
+
+// [5][neles] in original ids <-> [5][n] internal order on the device
+int upload_cells(tdt_ctx *c, const double *src, double *dst) {
+    const size_t n = c->pm.n;
+    std::vector<double> h(5 * n);
+    for (int q = 0; q < 5; ++q)
+        for (size_t i = 0; i < n; ++i) h[q * n + i] = src[q * n + c->pm.cell_new2old[i]];
+    HIPCALL(c, hipMemcpyAsync(dst, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice,
+                              c->stream), "upload");
+    return sync(c);
+}
+
+int download_cells(tdt_ctx *c, const double *src, double *dst) {
+    const size_t n = c->pm.n;
+    std::vector<double> h(5 * n);
+    HIPCALL(c, hipMemcpyAsync(h.data(), src, h.size() * sizeof(double), hipMemcpyDeviceToHost,
+                              c->stream), "download");
+    int rc = sync(c);
+    if (rc) return rc;
+    for (int q = 0; q < 5; ++q)
+        for (size_t i = 0; i < n; ++i) dst[q * n + c->pm.cell_new2old[i]] = h[q * n + i];
+    return TDT_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -362,6 +434,7 @@ int tdt_destroy(tdt_ctx *c) {
     for (void *p : c->allocs) hipFree(p);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c->batch;
+    delete c->adj;
     delete c;
     return TDT_OK;
 }
@@ -561,6 +634,62 @@ int tdt_kernel_set(tdt_ctx *c, int32_t *set) {
 int tdt_synchronize(tdt_ctx *c) {
     CHECK_CTX(c);
     return sync(c);
+}
+
+// ---- residual and its adjoint ------------------------------------------------------
+
+int tdt_set_params(tdt_ctx *c, const tdt_params *P) {
+    CHECK_SINGLE(c);
+    if (!P) return TDT_ERR_ARG;
+    if (!params_ok(*P)) {
+        c->err = g_create_error;
+        return TDT_ERR_ARG;
+    }
+    if (P->iaxis != c->params.iaxis) {
+        c->err = "tdt_set_params: IAXIS fixes the geometry and cannot change";
+        return TDT_ERR_UNSUPPORTED;
+    }
+    int rc = sync(c);
+    if (rc) return rc;
+    c->params = *P;
+    c->params_rtred = P->rtred;
+    c->K = make_consts<double>(*P);
+    c->Kf = make_consts<float>(*P);
+    set_rinit(c, c->rinit);
+    tdt::fused_free(c);   // its graphs hold the old constants; rebuilt on the next iterate
+    return TDT_OK;
+}
+
+int tdt_residual_eval(tdt_ctx *c, const double *cst, double *r) {
+    CHECK_SINGLE(c);
+    if (!cst || !r) return TDT_ERR_ARG;
+    int rc = adj_setup(c);
+    if (!rc) rc = upload_cells(c, cst, c->adj->q);
+    if (rc) return rc;
+    HIPCALL(c, tdt::launch_res_eval(c, c->adj->q, c->adj->r), "residual");
+    return download_cells(c, c->adj->r, r);
+}
+
+int tdt_residual_vjp(tdt_ctx *c, const double *cst, const double *v, double *gq,
+                     double *gparam) {
+    CHECK_SINGLE(c);
+    if (!cst || !v || !gq) return TDT_ERR_ARG;
+    int rc = adj_setup(c);
+    if (!rc) rc = upload_cells(c, cst, c->adj->q);
+    if (!rc) rc = upload_cells(c, v, c->adj->v);
+    if (rc) return rc;
+    tdt::AdjBufs &a = *c->adj;
+    HIPCALL(c, tdt::launch_res_vjp(c, a.q, a.v, a.tptr, a.tlist, a.J, a.Jp, a.part, a.g, a.gp),
+            "residual adjoint");
+    rc = download_cells(c, a.g, gq);
+    if (rc || !gparam) return rc;
+    double gp[2] = {0.0, 0.0};
+    HIPCALL(c, hipMemcpyAsync(gp, a.gp, sizeof gp, hipMemcpyDeviceToHost, c->stream), "download");
+    rc = sync(c);
+    gparam[0] = gp[0];
+    gparam[1] = gp[1];
+    gparam[2] = 0.0;   // R does not depend on AMACHINIT at a given state
+    return rc;
 }
 
 // ---- batched cases ----------------------------------------------------------------

@@ -70,6 +70,16 @@ struct BatchInfo {
     int4 *d_cblk = nullptr, *d_nblk = nullptr, *d_ccase = nullptr;
 };
 
+// Buffers of the residual adjoint (adjoint.hip), internal cell order: the
+// state, the weight and the results, the per-cell Jacobian blocks, and the
+// transposed stencil (for cell k, the (cell i, member s) whose member s is k).
+struct AdjBufs {
+    double *q = nullptr, *v = nullptr, *g = nullptr, *r = nullptr;
+    double *J = nullptr, *Jp = nullptr, *part = nullptr, *gp = nullptr;
+    int *tptr = nullptr;
+    int2 *tlist = nullptr;
+};
+
 }  // namespace tdt
 
 struct tdt_ctx {
@@ -91,6 +101,7 @@ struct tdt_ctx {
     int geom_version = 0;     // bumped when tdt_set_array changes geometry
     double *post = nullptr;   // [6][m] POSTPROCESS output, allocated on first use
     tdt::BatchInfo *batch = nullptr;   // tdt_batch_create contexts only
+    tdt::AdjBufs *adj = nullptr;       // residual adjoint buffers, allocated on first use
 };
 
 namespace tdt {
@@ -112,6 +123,12 @@ hipError_t launch_cold_start(tdt_ctx *c, double rho, double ru, double rv, doubl
 hipError_t launch_copy_cold(tdt_ctx *c);
 // tec_plot.dat node columns U, V, P [bar], RHO, MACH, T into out[6][m].
 hipError_t launch_postprocess(tdt_ctx *c, double *out);
+
+// Residual and its adjoint (adjoint.hip); qs, v, r, g: [5][n] internal order.
+hipError_t launch_res_eval(tdt_ctx *c, const double *qs, double *r);
+hipError_t launch_res_vjp(tdt_ctx *c, const double *qs, const double *v, const int *tptr,
+                          const int2 *tlist, double *J, double *Jp, double *part, double *g,
+                          double *gp);
 
 // Fused path (fused.hip).
 int fused_setup(tdt_ctx *c);

@@ -1,0 +1,57 @@
+"""torch.autograd binding of the residual and its adjoint.
+
+`Residual.apply(q, p, solver)` returns R(q) for the state q (float64 tensor
+[5, neles]: C1..C5 in grid.in cell order) with p = (P0, T0, AMACHINIT) as the
+flow parameters of `solver` (a tdt.Solver, whose grid and other flow.in values
+are used).  R is COMPUTE_RESIDUAL's R1..R5 after the primitive variables,
+boundary conditions and source terms of that state (tdt_residual_eval).  Its
+backward is the library's adjoint (tdt_residual_vjp): grad_q = v^T dR/dQ and
+grad_p = v^T dR/dp, computed on the GPU by the transposed stencil gather.
+
+The tensors stay on the CPU: the library holds the device buffers, so torch is
+the autograd plumbing here, not a compute path.  libtdt is loaded before torch
+so that both use the one HIP runtime the library was built against.
+"""
+from __future__ import annotations
+
+import dataclasses
+
+import numpy as np
+
+from . import lib
+
+lib.load()
+
+import torch  # noqa: E402  (after the library, see above)
+
+
+def _use_params(solver, p) -> None:
+    p0, t0, mach = (float(x) for x in p.detach().cpu().reshape(3))
+    f = solver.flow
+    if (f.p0, f.t0, f.amachinit) != (p0, t0, mach):
+        solver.set_flow(dataclasses.replace(f, p0=p0, t0=t0, amachinit=mach))
+
+
+class Residual(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, q, p, solver):
+        _use_params(solver, p)
+        r = solver.residual_of(q.detach().cpu().numpy().astype(np.float64))
+        ctx.solver = solver
+        ctx.save_for_backward(q, p)
+        return torch.from_numpy(r).to(dtype=q.dtype)
+
+    @staticmethod
+    def backward(ctx, grad_r):
+        q, p = ctx.saved_tensors
+        solver = ctx.solver
+        _use_params(solver, p)
+        gq, gp = solver.residual_vjp(q.detach().cpu().numpy().astype(np.float64),
+                                     grad_r.detach().cpu().numpy().astype(np.float64))
+        return (torch.from_numpy(gq).to(dtype=q.dtype), torch.from_numpy(gp).to(dtype=p.dtype),
+                None)
+
+
+def residual(q, p, solver):
+    """R(q; P0, T0, AMACHINIT) with the adjoint as its backward."""
+    return Residual.apply(q, p, solver)

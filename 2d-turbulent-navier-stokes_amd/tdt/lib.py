@@ -90,6 +90,9 @@ def load(path: str = LIB_PATH):
     L.tdt_kernel_times.argtypes = [vp, _dp]
     L.tdt_kernel_set.argtypes = [vp, _ip]
     L.tdt_postprocess.argtypes = [vp, _dp]
+    L.tdt_residual_eval.argtypes = [vp, _dp, _dp]
+    L.tdt_residual_vjp.argtypes = [vp, _dp, _dp, _dp, _dp]
+    L.tdt_set_params.argtypes = [vp, ctypes.POINTER(_Params)]
     L.tdt_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int32, ctypes.POINTER(_Mesh),
                                    ctypes.POINTER(_Params), ctypes.c_int32, ctypes.c_int32]
     L.tdt_batch_cold_start.argtypes = [vp, _dp]
@@ -228,6 +231,30 @@ class Solver:
             return l2, linf, bad.value
         self._check(rc)
         return l2, linf, bad.value
+
+    # ---- residual and its adjoint -----------------------------------------------------
+    def residual_of(self, c: np.ndarray) -> np.ndarray:
+        """R(c): COMPUTE_RESIDUAL's R1..R5 after primitives, boundary conditions and
+        source terms on the state c ([5, neles]); the context's state is untouched."""
+        c = np.ascontiguousarray(c, dtype=np.float64).reshape(5, self.neles)
+        r = np.empty((5, self.neles))
+        self._check(self.L.tdt_residual_eval(self.h, _d(c), _d(r)))
+        return r
+
+    def residual_vjp(self, c: np.ndarray, v: np.ndarray):
+        """(v^T dR/dQ [5, neles], v^T dR/d(P0, T0, AMACHINIT) [3]) at the state c."""
+        c = np.ascontiguousarray(c, dtype=np.float64).reshape(5, self.neles)
+        v = np.ascontiguousarray(v, dtype=np.float64).reshape(5, self.neles)
+        g = np.empty((5, self.neles))
+        gp = np.zeros(3)
+        self._check(self.L.tdt_residual_vjp(self.h, _d(c), _d(v), _d(g), _d(gp)))
+        return g, gp
+
+    def set_flow(self, flow: Flow) -> None:
+        """New flow.in values (same IAXIS) for this context."""
+        p = make_params(flow)
+        self._check(self.L.tdt_set_params(self.h, ctypes.byref(p)))
+        self.flow = flow
 
     # ---- arrays ----------------------------------------------------------------------
     def array_len(self, name: str) -> int:

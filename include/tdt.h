@@ -125,6 +125,25 @@ int tdt_postprocess(tdt_ctx *ctx, double *nodal);
 int tdt_iterate(tdt_ctx *ctx, int32_t niter, double *l2, double *linf,
                 int32_t *first_nonfinite);
 
+/* ---- residual and its adjoint ------------------------------------------------------
+ * No counterpart in the reference (which has no derivative code): the residual
+ * operator of one RK stage as a function, and its adjoint, for gradients.
+ * R(Q) = R1..R5 of COMPUTE_RESIDUAL (residual.f90:23-62) after
+ * COMPUTE_PRIMITIVE_VARS, APPLY_BOUNDARY and COMPUTE_SOURCE_TERMS on the state
+ * c = C1..C5 ([5][neles], grid.in cell order), with the context's geometry and
+ * flow constants; the context's own state is not touched.  tdt_residual_eval
+ * gives r = R(c) ([5][neles]; the same bits as the routine entry points).
+ * tdt_residual_vjp gives gq = v^T dR/dQ ([5][neles]) and gparam = v^T dR/d(P0,
+ * T0, AMACHINIT) ([3] or NULL; AMACHINIT enters R only through the initial
+ * state, so its entry is 0).  BC type 6 grids and batch contexts:
+ * TDT_ERR_UNSUPPORTED. */
+int tdt_residual_eval(tdt_ctx *ctx, const double *c, double *r);
+int tdt_residual_vjp(tdt_ctx *ctx, const double *c, const double *v, double *gq,
+                     double *gparam);
+/* New flow.in values for the context (same IAXIS): constants rebuilt, RINIT
+ * kept; the next tdt_iterate recaptures its graph. */
+int tdt_set_params(tdt_ctx *ctx, const tdt_params *params);
+
 /* ---- batched cases ----------------------------------------------------------------
  * No counterpart in the reference, which runs one case per process: for parameter
  * sweeps, ncases independent grids advanced together.  One fused iteration (the
