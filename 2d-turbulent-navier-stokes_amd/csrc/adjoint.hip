@@ -15,7 +15,12 @@
 //              whose member s is k, in increasing i, of v_i^T dR_i/dQ_s -- the
 //              transposed stencil, deterministic, no atomics;
 //   k_par_vjp  + k_par_fold: v^T dR/d(P0, T0) as block partial sums folded in a
-//              fixed order.
+//              fixed order;
+//   k_xy_vjp   v^T dR/d(node x, y): k_res_jac also differentiates R_i through the
+//              geometry of geometry.f90 recomputed from the nodes it reads
+//              (the cell's, its face neighbours', and the ghost centroids'
+//              face nodes), and the gather runs over each node's (cell, slot)
+//              list.
 #include "ctx.hpp"
 #include "dual.hpp"
 #include "reduce.hpp"
@@ -26,6 +31,95 @@ namespace {
 constexpr int kJacBlocks = 5;         // stencil members per cell: self, faces 1..4
 constexpr int kJacStride = 5 * 5 * kJacBlocks;
 constexpr int kDualN = 5;             // directions per dual evaluation
+// node slots per cell: the cell's four nodes, then each face neighbour's four
+// (a node shared by several slots collects the sum of their derivatives)
+constexpr int kNodeSlots = 20;
+constexpr int kXyStride = kNodeSlots * 2 * 5;   // [slot][x, y][m]
+
+// Geometry a cell's residual reads: its surface vectors and volume, and the
+// centroids of the cell (index 0) and of the cell or ghost across each face.
+template <class R>
+struct CellGeo {
+    R sc[8];
+    R vol;
+    R xcc[5], ycc[5];
+};
+
+// ... as stored (geometry.f90 arithmetic, computed once at set-up)
+template <class R>
+__device__ __forceinline__ void geo_stored(const Dev &D, int i, CellGeo<R> &G) {
+    const int n = D.n;
+    const int4 nb4 = D.nc4[i];
+    const int nb[4] = {nb4.x, nb4.y, nb4.z, nb4.w};
+    for (int k = 0; k < 8; ++k) G.sc[k] = R(D.sc[k * n + i]);
+    G.vol = R(D.vol[i]);
+    G.xcc[0] = R(D.xcc[i]);
+    G.ycc[0] = R(D.ycc[i]);
+    for (int f = 0; f < 4; ++f) {
+        G.xcc[1 + f] = R(D.xcc[nb[f]]);
+        G.ycc[1 + f] = R(D.ycc[nb[f]]);
+    }
+}
+
+// ... from node coordinates (metres) with k_geometry's and k_geometry_ghosts'
+// expressions: x[s], y[s] for the node slots (0-3: the cell's nodes 1-4,
+// 4 + 4f + k: node k+1 of the cell across face f+1; unused on ghost faces).
+template <class R>
+__device__ void geo_from_nodes(const Dev &D, int i, const R (&x)[kNodeSlots],
+                               const R (&y)[kNodeSlots], int iaxis, CellGeo<R> &G) {
+    const int n = D.n;
+    const int4 nb4 = D.nc4[i];
+    const int nb[4] = {nb4.x, nb4.y, nb4.z, nb4.w};
+    const R x1 = x[0], x2 = x[1], x3 = x[2], x4 = x[3];
+    const R y1 = y[0], y2 = y[1], y3 = y[2], y4 = y[3];
+    G.xcc[0] = (x1 + x2 + x3 + x4) * R(0.25);
+    G.ycc[0] = (y1 + y2 + y3 + y4) * R(0.25);
+    R vol;
+    if (iaxis == 1) {
+        G.sc[0] = (y2 - y1) * (y1 + y2) * R(0.5);
+        G.sc[1] = -((x2 - x1) * (y1 + y2) * R(0.5));
+        G.sc[2] = (y3 - y2) * (y2 + y3) * R(0.5);
+        G.sc[3] = -((x3 - x2) * (y2 + y3) * R(0.5));
+        G.sc[4] = (y4 - y3) * (y3 + y4) * R(0.5);
+        G.sc[5] = -((x4 - x3) * (y3 + y4) * R(0.5));
+        G.sc[6] = (y1 - y4) * (y4 + y1) * R(0.5);
+        G.sc[7] = -((x1 - x4) * (y4 + y1) * R(0.5));
+        vol = R(0.5) * ((x3 - x1) * (y4 - y2) - (x4 - x2) * (y3 - y1)) * G.ycc[0];
+    } else {
+        G.sc[0] = (y2 - y1);
+        G.sc[1] = -(x2 - x1);
+        G.sc[2] = (y3 - y2);
+        G.sc[3] = -(x3 - x2);
+        G.sc[4] = (y4 - y3);
+        G.sc[5] = -(x4 - x3);
+        G.sc[6] = (y1 - y4);
+        G.sc[7] = -(x1 - x4);
+        vol = R(0.5) * ((x3 - x1) * (y4 - y2) - (x4 - x2) * (y3 - y1));
+    }
+    if (vol <= R(0)) vol = -vol;
+    G.vol = vol;
+    for (int f = 0; f < 4; ++f) {
+        if (nb[f] < n) {
+            const int b = 4 + 4 * f;
+            G.xcc[1 + f] = (x[b] + x[b + 1] + x[b + 2] + x[b + 3]) * R(0.25);
+            G.ycc[1 + f] = (y[b] + y[b + 1] + y[b + 2] + y[b + 3]) * R(0.25);
+        } else {   // ghost: the centroid mirrored through the face midpoint
+            const int a = f, c = (f + 1) & 3;
+            const R xf = R(0.5) * (x[a] + x[c]);
+            const R yf = R(0.5) * (y[a] + y[c]);
+            G.xcc[1 + f] = R(2.0) * xf - G.xcc[0];
+            G.ycc[1 + f] = R(2.0) * yf - G.ycc[0];
+        }
+    }
+}
+
+// node ids of the slots of cell i (-1 for the slots of a ghost face)
+__device__ __forceinline__ int slot_node(const Dev &D, int i, int s) {
+    if (s < 4) return reinterpret_cast<const int *>(D.nod4)[4 * i + s];
+    const int f = (s - 4) >> 2, k = (s - 4) & 3;
+    const int j = reinterpret_cast<const int *>(D.nc4)[4 * i + f];
+    return j < D.n ? reinterpret_cast<const int *>(D.nod4)[4 * j + k] : -1;
+}
 
 __device__ __forceinline__ int gid() { return blockIdx.x * kBlock + threadIdx.x; }
 
@@ -33,8 +127,8 @@ __device__ __forceinline__ int gid() { return blockIdx.x * kBlock + threadIdx.x;
 // unused for a ghost face), with the same operations in the same order as
 // k_primitives, k_bc, k_source and k_residual.
 template <class R>
-__device__ void cell_residual(const Dev &D, int i, const R (&q)[5][5], const Consts<R> &K,
-                              R (&r)[5]) {
+__device__ void cell_residual(const Dev &D, int i, const R (&q)[5][5], const CellGeo<R> &G,
+                              const Consts<R> &K, R (&r)[5]) {
     const int n = D.n;
     const int4 nb4 = D.nc4[i];
     const int nb[4] = {nb4.x, nb4.y, nb4.z, nb4.w};
@@ -51,19 +145,17 @@ __device__ void cell_residual(const Dev &D, int i, const R (&q)[5][5], const Con
         if (ty == 3) {   // the parent's opposite neighbour: a cell (checked at set-up)
             uo = pj[o].u; vo = pj[o].v; to = pj[o].t; po = pj[o].p;
         }
-        ghost_state<R>(pj[f], ty, pi, uo, vo, to, po, R(D.ycc[nb[f]]), K);
+        ghost_state<R>(pj[f], ty, pi, uo, vo, to, po, G.ycc[1 + f], K);
     }
     // COMPUTE_SOURCE_TERMS
     R s3;
     auto nbr = [&](int f) {
-        return src_nbr<R>(R(D.xcc[nb[f]]), R(D.ycc[nb[f]]), pj[f].u, pj[f].v, pj[f].c[0],
-                          pj[f].c[4]);
+        return src_nbr<R>(G.xcc[1 + f], G.ycc[1 + f], pj[f].u, pj[f].v, pj[f].c[0], pj[f].c[4]);
     };
     const R s5 = source_terms<R>(nbr(0), nbr(1), nbr(2), nbr(3), pi.c[0], pi.c[4], pi.t, pi.p,
-                                 pi.v, R(D.ycc[i]), s3, K);
+                                 pi.v, G.ycc[0], s3, K);
     // COMPUTE_RESIDUAL
-    R sc[8];
-    for (int k = 0; k < 8; ++k) sc[k] = R(D.sc[k * n + i]);
+    const R(&sc)[8] = G.sc;
     for (int m = 0; m < 5; ++m) {
         R s = (pi.f[m] + pj[0].f[m]) * sc[0] + (pi.g[m] + pj[0].g[m]) * sc[1];
         for (int k = 1; k < 4; ++k) {
@@ -72,8 +164,8 @@ __device__ void cell_residual(const Dev &D, int i, const R (&q)[5][5], const Con
         }
         r[m] = R(0.5) * s;
     }
-    r[4] = r[4] - R(D.vol[i]) * s5;
-    if (K.iaxis == 1) r[2] = r[2] - R(D.vol[i]) * s3;
+    r[4] = r[4] - G.vol * s5;
+    if (K.iaxis == 1) r[2] = r[2] - G.vol * s3;
 }
 
 // The stencil states of cell i from qs ([5][n], internal order): q[0] = cell,
@@ -96,7 +188,9 @@ __global__ void __launch_bounds__(kBlock) k_res_eval(Dev D, Consts<double> K, co
     if (i >= D.n) return;
     double q[5][5], ri[5];
     load_stencil(D, qs, i, q);
-    cell_residual<double>(D, i, q, K, ri);
+    CellGeo<double> G;
+    geo_stored(D, i, G);
+    cell_residual<double>(D, i, q, G, K, ri);
     for (int m = 0; m < 5; ++m) r[(size_t)m * D.n + i] = ri[m];
 }
 
@@ -110,7 +204,8 @@ struct ParamSeeds {
 // J[i] = [s][m][c]: dR_i,m / dQ_s,c (zero blocks for ghost faces); Jp[i] =
 // [m][p]: dR_i,m / d(P0, T0).
 __global__ void __launch_bounds__(kBlock) k_res_jac(Dev D, Consts<double> K, ParamSeeds ps,
-                                                    const double *qs, double *J, double *Jp) {
+                                                    const double *qs, double *J, double *Jp,
+                                                    double *Jx) {
     const int i = gid();
     if (i >= D.n) return;
     const int n = D.n;
@@ -119,6 +214,8 @@ __global__ void __launch_bounds__(kBlock) k_res_jac(Dev D, Consts<double> K, Par
     double qd[5][5];
     load_stencil(D, qs, i, qd);
     const Consts<Dual<kDualN>> KD = lift<kDualN>(K);
+    CellGeo<Dual<kDualN>> GD;
+    geo_stored(D, i, GD);
     double *Ji = J + (size_t)i * kJacStride;
     for (int s = 0; s < kJacBlocks; ++s) {
         if (s > 0 && nb[s - 1] >= n) {
@@ -129,7 +226,7 @@ __global__ void __launch_bounds__(kBlock) k_res_jac(Dev D, Consts<double> K, Par
         for (int a = 0; a < 5; ++a)
             for (int c = 0; c < 5; ++c) q[a][c] = Dual<kDualN>(qd[a][c]);
         for (int c = 0; c < 5; ++c) q[s][c].d[c] = 1.0;   // direction c: Q_s,c
-        cell_residual<Dual<kDualN>>(D, i, q, KD, r);
+        cell_residual<Dual<kDualN>>(D, i, q, GD, KD, r);
         for (int m = 0; m < 5; ++m)
             for (int c = 0; c < 5; ++c) Ji[s * 25 + m * 5 + c] = r[m].d[c];
     }
@@ -143,11 +240,64 @@ __global__ void __launch_bounds__(kBlock) k_res_jac(Dev D, Consts<double> K, Par
     Dual<kDualN> q[5][5], r[5];
     for (int a = 0; a < 5; ++a)
         for (int c = 0; c < 5; ++c) q[a][c] = Dual<kDualN>(qd[a][c]);
-    cell_residual<Dual<kDualN>>(D, i, q, KP, r);
+    cell_residual<Dual<kDualN>>(D, i, q, GD, KP, r);
     for (int m = 0; m < 5; ++m) {
         Jp[(size_t)i * 10 + m * 2] = r[m].d[0];
         Jp[(size_t)i * 10 + m * 2 + 1] = r[m].d[1];
     }
+    if (!Jx) return;
+    // node coordinates: 40 directions (slot, x or y), kDualN at a time
+    double xs[kNodeSlots], ys[kNodeSlots];
+    for (int sl = 0; sl < kNodeSlots; ++sl) {
+        const int v = slot_node(D, i, sl);
+        xs[sl] = v >= 0 ? D.xn[v] : 0.0;
+        ys[sl] = v >= 0 ? D.yn[v] : 0.0;
+    }
+    double *Jxi = Jx + (size_t)i * kXyStride;
+    for (int d0 = 0; d0 < 2 * kNodeSlots; d0 += kDualN) {
+        Dual<kDualN> x[kNodeSlots], y[kNodeSlots];
+        for (int sl = 0; sl < kNodeSlots; ++sl) {
+            x[sl] = Dual<kDualN>(xs[sl]);
+            y[sl] = Dual<kDualN>(ys[sl]);
+        }
+        for (int e = 0; e < kDualN; ++e) {   // direction e: slot (d0+e)/2, x if even
+            const int dir = d0 + e, sl = dir >> 1;
+            if (dir & 1) y[sl].d[e] = 1.0;
+            else x[sl].d[e] = 1.0;
+        }
+        CellGeo<Dual<kDualN>> GX;
+        geo_from_nodes(D, i, x, y, K.iaxis, GX);
+        cell_residual<Dual<kDualN>>(D, i, q, GX, KD, r);
+        for (int e = 0; e < kDualN; ++e) {
+            const int dir = d0 + e;
+            for (int m = 0; m < 5; ++m) Jxi[dir * 5 + m] = r[m].d[e];
+        }
+    }
+}
+
+// gxy_v = sum over the (cell i, slot s) whose slot holds node v, in order, of
+// v_i^T dR_i/d(x, y)_s
+__global__ void __launch_bounds__(kBlock) k_xy_vjp(Dev D, const double *Jx, const double *v,
+                                                   const int *nptr, const int2 *nlist,
+                                                   double *gxy) {
+    const int nd = gid();
+    if (nd >= D.m) return;
+    const int n = D.n;
+    double ax = 0.0, ay = 0.0;
+    for (int e = nptr[nd]; e < nptr[nd + 1]; ++e) {
+        const int2 is = nlist[e];
+        const double *B = Jx + (size_t)is.x * kXyStride + is.y * 10;
+        double tx = v[is.x] * B[0], ty = v[is.x] * B[5];
+        for (int m = 1; m < 5; ++m) {
+            const double vm = v[(size_t)m * n + is.x];
+            tx = tx + vm * B[m];
+            ty = ty + vm * B[5 + m];
+        }
+        ax = ax + tx;
+        ay = ay + ty;
+    }
+    gxy[nd] = ax;
+    gxy[D.m + nd] = ay;
 }
 
 // g_k = sum over the entries (i, s) of cell k's transposed stencil, in order,
@@ -217,7 +367,8 @@ hipError_t launch_res_eval(tdt_ctx *c, const double *qs, double *r) {
 
 hipError_t launch_res_vjp(tdt_ctx *c, const double *qs, const double *v, const int *tptr,
                           const int2 *tlist, double *J, double *Jp, double *part, double *g,
-                          double *gp) {
+                          double *gp, double *Jx, const int *nptr, const int2 *nlist,
+                          double *gxy) {
     const Consts<double> &K = c->K;
     const tdt_params &P = c->params;
     ParamSeeds ps;
@@ -227,11 +378,15 @@ hipError_t launch_res_vjp(tdt_ctx *c, const double *qs, const double *v, const i
     ps.dtinf_dt0 = (double)K.tinf / P.t0;
     ps.dblu_dt0 = 0.5 * (double)K.bl_u / P.t0;
     const unsigned nb = nblocks(c->D.n);
-    hipLaunchKernelGGL(k_res_jac, dim3(nb), dim3(kBlock), 0, c->stream, c->D, K, ps, qs, J, Jp);
+    hipLaunchKernelGGL(k_res_jac, dim3(nb), dim3(kBlock), 0, c->stream, c->D, K, ps, qs, J, Jp,
+                       gxy ? Jx : nullptr);
     hipLaunchKernelGGL(k_res_vjp, dim3(nb), dim3(kBlock), 0, c->stream, c->D, J, v, tptr, tlist,
                        g);
     hipLaunchKernelGGL(k_par_vjp, dim3(nb), dim3(kBlock), 0, c->stream, c->D, Jp, v, part);
     hipLaunchKernelGGL(k_par_fold, dim3(1), dim3(kWave), 0, c->stream, part, (int)nb, gp);
+    if (gxy)
+        hipLaunchKernelGGL(k_xy_vjp, dim3(nblocks(c->D.m)), dim3(kBlock), 0, c->stream, c->D, Jx,
+                           v, nptr, nlist, gxy);
     return hipGetLastError();
 }
 

@@ -78,6 +78,11 @@ struct AdjBufs {
     double *J = nullptr, *Jp = nullptr, *part = nullptr, *gp = nullptr;
     int *tptr = nullptr;
     int2 *tlist = nullptr;
+    // node coordinates: per-cell blocks over its 20 node slots, the per-node
+    // (cell, slot) lists and the result [2][m]
+    double *Jx = nullptr, *gxy = nullptr;
+    int *nptr = nullptr;
+    int2 *nlist = nullptr;
 };
 
 }  // namespace tdt
@@ -128,7 +133,8 @@ hipError_t launch_postprocess(tdt_ctx *c, double *out);
 hipError_t launch_res_eval(tdt_ctx *c, const double *qs, double *r);
 hipError_t launch_res_vjp(tdt_ctx *c, const double *qs, const double *v, const int *tptr,
                           const int2 *tlist, double *J, double *Jp, double *part, double *g,
-                          double *gp);
+                          double *gp, double *Jx, const int *nptr, const int2 *nlist,
+                          double *gxy);
 
 // Fused path (fused.hip).
 int fused_setup(tdt_ctx *c);

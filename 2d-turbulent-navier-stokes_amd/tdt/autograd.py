@@ -1,12 +1,14 @@
 """torch.autograd binding of the residual and its adjoint.
 
-`Residual.apply(q, p, solver)` returns R(q) for the state q (float64 tensor
+`residual(q, p, solver, xy)` returns R(q) for the state q (float64 tensor
 [5, neles]: C1..C5 in grid.in cell order) with p = (P0, T0, AMACHINIT) as the
-flow parameters of `solver` (a tdt.Solver, whose grid and other flow.in values
-are used).  R is COMPUTE_RESIDUAL's R1..R5 after the primitive variables,
-boundary conditions and source terms of that state (tdt_residual_eval).  Its
-backward is the library's adjoint (tdt_residual_vjp): grad_q = v^T dR/dQ and
-grad_p = v^T dR/dp, computed on the GPU by the transposed stencil gather.
+flow parameters and xy ([2, nods], grid.in millimetres; optional) as the node
+coordinates of `solver` (a tdt.Solver, whose connectivity and other flow.in
+values are used).  R is COMPUTE_RESIDUAL's R1..R5 after the primitive
+variables, boundary conditions and source terms of that state
+(tdt_residual_eval).  Its backward is the library's adjoint
+(tdt_residual_vjp): v^T dR/dQ, v^T dR/dp and v^T dR/d(X, Y), computed on the GPU
+by transposed stencil gathers.
 
 The tensors stay on the CPU: the library holds the device buffers, so torch is
 the autograd plumbing here, not a compute path.  libtdt is loaded before torch
@@ -32,26 +34,38 @@ def _use_params(solver, p) -> None:
         solver.set_flow(dataclasses.replace(f, p0=p0, t0=t0, amachinit=mach))
 
 
+def _use_nodes(solver, xy) -> None:
+    a = xy.detach().cpu().numpy().astype(np.float64).reshape(2, -1)
+    if not np.array_equal(a, solver.xy_mm):
+        solver.set_nodes(a[0], a[1])
+
+
 class Residual(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, q, p, solver):
+    def forward(ctx, q, p, xy, solver):
         _use_params(solver, p)
+        _use_nodes(solver, xy)
         r = solver.residual_of(q.detach().cpu().numpy().astype(np.float64))
         ctx.solver = solver
-        ctx.save_for_backward(q, p)
+        ctx.save_for_backward(q, p, xy)
         return torch.from_numpy(r).to(dtype=q.dtype)
 
     @staticmethod
     def backward(ctx, grad_r):
-        q, p = ctx.saved_tensors
+        q, p, xy = ctx.saved_tensors
         solver = ctx.solver
         _use_params(solver, p)
-        gq, gp = solver.residual_vjp(q.detach().cpu().numpy().astype(np.float64),
-                                     grad_r.detach().cpu().numpy().astype(np.float64))
+        _use_nodes(solver, xy)
+        gq, gp, gxy = solver.residual_vjp(q.detach().cpu().numpy().astype(np.float64),
+                                          grad_r.detach().cpu().numpy().astype(np.float64),
+                                          nodes=True)
         return (torch.from_numpy(gq).to(dtype=q.dtype), torch.from_numpy(gp).to(dtype=p.dtype),
-                None)
+                torch.from_numpy(gxy).to(dtype=xy.dtype), None)
 
 
-def residual(q, p, solver):
-    """R(q; P0, T0, AMACHINIT) with the adjoint as its backward."""
-    return Residual.apply(q, p, solver)
+def residual(q, p, solver, xy=None):
+    """R(q; X, Y, P0, T0, AMACHINIT) with the adjoint as its backward; xy defaults
+    to the solver's current node coordinates."""
+    if xy is None:
+        xy = torch.from_numpy(solver.xy_mm.copy())
+    return Residual.apply(q, p, xy, solver)

@@ -91,7 +91,8 @@ def load(path: str = LIB_PATH):
     L.tdt_kernel_set.argtypes = [vp, _ip]
     L.tdt_postprocess.argtypes = [vp, _dp]
     L.tdt_residual_eval.argtypes = [vp, _dp, _dp]
-    L.tdt_residual_vjp.argtypes = [vp, _dp, _dp, _dp, _dp]
+    L.tdt_residual_vjp.argtypes = [vp, _dp, _dp, _dp, _dp, _dp]
+    L.tdt_set_nodes.argtypes = [vp, _dp, _dp]
     L.tdt_set_params.argtypes = [vp, ctypes.POINTER(_Params)]
     L.tdt_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int32, ctypes.POINTER(_Mesh),
                                    ctypes.POINTER(_Params), ctypes.c_int32, ctypes.c_int32]
@@ -161,6 +162,7 @@ class Solver:
         self.neles, self.nghosts, self.nods, self.nmax = info[0], info[1], info[2], info[3]
         self.bandwidth, self.ordering_used = info[4], info[6]
         self.precision = precision
+        self.xy_mm = np.stack([mesh.x_mm, mesh.y_mm]).astype(np.float64)
 
     def close(self):
         if getattr(self, "h", None):
@@ -241,14 +243,24 @@ class Solver:
         self._check(self.L.tdt_residual_eval(self.h, _d(c), _d(r)))
         return r
 
-    def residual_vjp(self, c: np.ndarray, v: np.ndarray):
-        """(v^T dR/dQ [5, neles], v^T dR/d(P0, T0, AMACHINIT) [3]) at the state c."""
+    def residual_vjp(self, c: np.ndarray, v: np.ndarray, nodes: bool = False):
+        """(v^T dR/dQ [5, neles], v^T dR/d(P0, T0, AMACHINIT) [3]) at the state c,
+        and with nodes=True also v^T dR/d(X, Y) [2, nods] per grid.in millimetre."""
         c = np.ascontiguousarray(c, dtype=np.float64).reshape(5, self.neles)
         v = np.ascontiguousarray(v, dtype=np.float64).reshape(5, self.neles)
         g = np.empty((5, self.neles))
         gp = np.zeros(3)
-        self._check(self.L.tdt_residual_vjp(self.h, _d(c), _d(v), _d(g), _d(gp)))
-        return g, gp
+        gxy = np.empty((2, self.nods)) if nodes else None
+        self._check(self.L.tdt_residual_vjp(self.h, _d(c), _d(v), _d(g), _d(gp),
+                                            _d(gxy) if nodes else None))
+        return (g, gp, gxy) if nodes else (g, gp)
+
+    def set_nodes(self, x_mm: np.ndarray, y_mm: np.ndarray) -> None:
+        """New node coordinates (grid.in order, mm); the geometry is recomputed."""
+        x = np.ascontiguousarray(x_mm, dtype=np.float64).reshape(self.nods)
+        y = np.ascontiguousarray(y_mm, dtype=np.float64).reshape(self.nods)
+        self._check(self.L.tdt_set_nodes(self.h, _d(x), _d(y)))
+        self.xy_mm = np.stack([x, y])
 
     def set_flow(self, flow: Flow) -> None:
         """New flow.in values (same IAXIS) for this context."""

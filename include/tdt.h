@@ -135,14 +135,18 @@ int tdt_iterate(tdt_ctx *ctx, int32_t niter, double *l2, double *linf,
  * gives r = R(c) ([5][neles]; the same bits as the routine entry points).
  * tdt_residual_vjp gives gq = v^T dR/dQ ([5][neles]) and gparam = v^T dR/d(P0,
  * T0, AMACHINIT) ([3] or NULL; AMACHINIT enters R only through the initial
- * state, so its entry is 0).  BC type 6 grids and batch contexts:
- * TDT_ERR_UNSUPPORTED. */
+ * state, so its entry is 0) and gxy = v^T dR/d(X, Y) ([2][nods] or NULL, per
+ * grid.in node and millimetre, through the geometry of geometry.f90:55-149).
+ * BC type 6 grids and batch contexts: TDT_ERR_UNSUPPORTED. */
 int tdt_residual_eval(tdt_ctx *ctx, const double *c, double *r);
 int tdt_residual_vjp(tdt_ctx *ctx, const double *c, const double *v, double *gq,
-                     double *gparam);
+                     double *gparam, double *gxy);
 /* New flow.in values for the context (same IAXIS): constants rebuilt, RINIT
  * kept; the next tdt_iterate recaptures its graph. */
 int tdt_set_params(tdt_ctx *ctx, const tdt_params *params);
+/* New node coordinates (grid.in order, millimetres; same connectivity): the
+ * geometry (geometry.f90:55-149) is recomputed. */
+int tdt_set_nodes(tdt_ctx *ctx, const double *x_mm, const double *y_mm);
 
 /* ---- batched cases ----------------------------------------------------------------
  * No counterpart in the reference, which runs one case per process: for parameter

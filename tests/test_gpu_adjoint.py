@@ -5,7 +5,7 @@ source terms.  Its value must be the routine path's, bit for bit; its adjoint
 v^T dR/dQ (the transposed stencil gather of the per-cell dual-number blocks)
 must agree with central finite differences of R along random directions, both
 of this library's R and of the CPU oracle's (the reference's arithmetic
-restated), and so must v^T dR/d(P0, T0).  States are the fixtures' final
+restated), and so must v^T dR/d(P0, T0) and v^T dR/d(node X, Y).  States are the fixtures' final
 states (developed flows: at a uniform cold start the turbulence production is
 exactly 0 and its square root has no derivative).
 """
@@ -129,6 +129,47 @@ def test_parameter_adjoint_matches_finite_differences(golden, case):
             assert abs(fd - gp[k]) <= 1e-5 * abs(gp[k]) + 1e-12, (name, fd, gp[k])
 
 
+def node_scale(mesh):
+    """Per node, its shortest adjacent cell edge (mm)."""
+    x, y = mesh.x_mm, mesh.y_mm
+    h = np.full(mesh.nods, np.inf)
+    for f in range(4):
+        a, b = mesh.nod[f] - 1, mesh.nod[(f + 1) % 4] - 1
+        d = np.hypot(x[a] - x[b], y[a] - y[b])
+        np.minimum.at(h, a, d)
+        np.minimum.at(h, b, d)
+    return h
+
+
+@pytest.mark.parametrize("case", ["t2d_chan", "taxi_duct", "t7_plate", "case1_axi"])
+def test_node_adjoint_matches_finite_differences(golden, case):
+    """v^T dR/d(X, Y) through the geometry (surface vectors, volumes, cell and
+    ghost centroids) against central differences of R with moved nodes."""
+    import tdt
+    g = golden(case)
+    c = g["final"]
+    rng = np.random.default_rng(7)
+    x0, y0 = g.mesh.x_mm.astype(float), g.mesh.y_mm.astype(float)
+    sc = node_scale(g.mesh)
+    with tdt.Solver(g.mesh, g.flow) as s:
+        s.cold_start()
+        for trial in range(2):
+            v = rng.standard_normal(c.shape)
+            wx = sc * rng.standard_normal(g.mesh.nods)
+            wy = sc * rng.standard_normal(g.mesh.nods)
+            gq, gp, gxy = s.residual_vjp(c, v, nodes=True)
+            h = 1e-7
+            r = []
+            for sign in (1.0, -1.0):
+                s.set_nodes(x0 + sign * h * wx, y0 + sign * h * wy)
+                r.append(s.residual_of(c))
+            s.set_nodes(x0, y0)
+            fd = float(np.sum(v * (r[0] - r[1]))) / (2.0 * h)
+            check_fd(fd, gxy, np.stack([wx, wy]))
+        # moving the nodes back restores the geometry bit for bit
+        assert np.array_equal(s.get_array("VOL"), tdt.Solver(g.mesh, g.flow).get_array("VOL"))
+
+
 TORCH_CHECK = r"""
 import sys
 import numpy as np
@@ -148,12 +189,14 @@ with tdt.Solver(mesh, flow) as s:
     v = rng.standard_normal(c.shape)
     q = torch.tensor(c, requires_grad=True)
     p = torch.tensor([flow.p0, flow.t0, flow.amachinit], dtype=torch.float64, requires_grad=True)
-    out = residual(q, p, s)
+    xy = torch.tensor(np.stack([mesh.x_mm, mesh.y_mm]), requires_grad=True)
+    out = residual(q, p, s, xy)
     assert np.array_equal(out.detach().numpy(), s.residual_of(c))
     (out * torch.from_numpy(v)).sum().backward()
-    gq, gp = s.residual_vjp(c, v)
+    gq, gp, gxy = s.residual_vjp(c, v, nodes=True)
     assert np.array_equal(q.grad.numpy(), gq), "grad q"
     assert np.array_equal(p.grad.numpy(), gp), "grad p"
+    assert np.array_equal(xy.grad.numpy(), gxy), "grad xy"
     # a second use with other parameters rebuilds the flow constants
     p2 = torch.tensor([1.1 * flow.p0, flow.t0, flow.amachinit], dtype=torch.float64)
     r2 = residual(torch.tensor(c), p2, s).numpy()
