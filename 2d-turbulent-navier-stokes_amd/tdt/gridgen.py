@@ -233,36 +233,38 @@ def bl_inlet_params(flow: Flow, deltabl_m: float = 0.005) -> Flow:
                         "deltabl": deltabl_m})
 
 
+def sweep_case(i: int, seed: int = DEFAULT_SEED, nx_range=(65, 1025), ny_range=(33, 257)):
+    """Case i of the batched parameter sweep, drawn from numpy's
+    default_rng([seed, i]): family (the 97x49 channel's or the 2049x513 ramp's
+    geometry, boundary types and flow) with equal odds, Nx and Ny log-uniform
+    over nx_range / ny_range, AMACHINIT uniform in [0.2, 2.0], P0 uniform in
+    [1e5, 2e6] Pa, T0 uniform in [250, 400] K; CFL 0.25, 5000 iterations.
+    Channel cases get the BC type-7 inputs of bl_inlet_params.  A ramp case
+    keeps the 2049-column ramp's node displacement in x (0.5 % of its spacing):
+    0.5 % of a coarser spacing would move the first node row across the sloped
+    wall, whose first cell is 0.002 mm tall, and such grids diverge at once in
+    the reference's arithmetic.  Returns (family, Mesh, Flow)."""
+    rng = np.random.default_rng([seed, i])
+    family = "channel" if rng.random() < 0.5 else "ramp"
+    nx = int(round(math.exp(rng.uniform(math.log(nx_range[0]), math.log(nx_range[1])))))
+    ny = int(round(math.exp(rng.uniform(math.log(ny_range[0]), math.log(ny_range[1])))))
+    mach = float(rng.uniform(0.2, 2.0))
+    p0 = float(rng.uniform(1e5, 2e6))
+    t0 = float(rng.uniform(250.0, 400.0))
+    case_seed = int(rng.integers(0, 2 ** 31 - 1))
+    if family == "channel":
+        mesh = channel(nx=nx, ny=ny, seed=case_seed)
+        flow = bl_inlet_params(flow_channel(amachinit=mach, p0=p0, t0=t0))
+    else:
+        mesh = ramp(nx=nx, ny=ny, seed=case_seed, jitter=0.005 * min(1.0, nx / 2049))
+        flow = flow_ramp(amachinit=mach, p0=p0, t0=t0)
+    return family, mesh, _with(flow, {"niter": 5000})
+
+
 def sweep_cases(ncases: int = 64, seed: int = DEFAULT_SEED, nx_range=(65, 1025),
                 ny_range=(33, 257)):
-    """The batched parameter sweep: ncases independent cases, case i drawn from
-    numpy's default_rng([seed, i]): family (the 97x49 channel's or the 2049x513
-    ramp's geometry, boundary types and flow) with equal odds, Nx and Ny
-    log-uniform over nx_range / ny_range, AMACHINIT uniform in [0.2, 2.0], P0
-    uniform in [1e5, 2e6] Pa, T0 uniform in [250, 400] K; CFL 0.25.  Channel
-    cases get the BC type-7 inputs of bl_inlet_params.  A ramp case keeps the
-    2049-column ramp's node displacement in x (0.5 % of its spacing): 0.5 % of a
-    coarser spacing would move the first node row across the sloped wall, whose
-    first cell is 0.002 mm tall, and such grids diverge at once in the reference's
-    arithmetic.  Returns a list of (family, Mesh, Flow)."""
-    out = []
-    for i in range(ncases):
-        rng = np.random.default_rng([seed, i])
-        family = "channel" if rng.random() < 0.5 else "ramp"
-        nx = int(round(math.exp(rng.uniform(math.log(nx_range[0]), math.log(nx_range[1])))))
-        ny = int(round(math.exp(rng.uniform(math.log(ny_range[0]), math.log(ny_range[1])))))
-        mach = float(rng.uniform(0.2, 2.0))
-        p0 = float(rng.uniform(1e5, 2e6))
-        t0 = float(rng.uniform(250.0, 400.0))
-        case_seed = int(rng.integers(0, 2 ** 31 - 1))
-        if family == "channel":
-            mesh = channel(nx=nx, ny=ny, seed=case_seed)
-            flow = bl_inlet_params(flow_channel(amachinit=mach, p0=p0, t0=t0))
-        else:
-            mesh = ramp(nx=nx, ny=ny, seed=case_seed, jitter=0.005 * min(1.0, nx / 2049))
-            flow = flow_ramp(amachinit=mach, p0=p0, t0=t0)
-        out.append((family, mesh, _with(flow, {"niter": 5000})))
-    return out
+    """The batched parameter sweep: sweep_case(i) for i < ncases."""
+    return [sweep_case(i, seed, nx_range, ny_range) for i in range(ncases)]
 
 
 def _with(f: Flow, kw: dict) -> Flow:

@@ -245,7 +245,7 @@ def large_case(stream_gbs, steps=200, warmup=50):
 
 
 def batch_sweep(ncases=64, steps=100, warmup=20, seq_steps=50, precision="fp32-mixed",
-                sequential=True):
+                sequential=True, ordering="auto"):
     """The batched parameter sweep (gridgen.sweep_cases: 64 cases of both
     families, uneven sizes): all cases in one batch context, one fused iteration
     per step over the concatenated cells, against the same cases run one after
@@ -255,7 +255,7 @@ def batch_sweep(ncases=64, steps=100, warmup=20, seq_steps=50, precision="fp32-m
     sizes = [m.neles for _, m, _ in cases]
     ntot = sum(sizes)
     pairs = [(m, f) for _, m, f in cases]
-    with tdt.Batch(pairs, precision) as b:
+    with tdt.Batch(pairs, precision, ordering) as b:
         b.cold_start()
         b.iterate(warmup, history=False, check=False)
         b.synchronize()
@@ -271,7 +271,7 @@ def batch_sweep(ncases=64, steps=100, warmup=20, seq_steps=50, precision="fp32-m
     out = {"workload": f"sweep of {ncases} cases ({fam.count('channel')} channel, "
                        f"{fam.count('ramp')} ramp), {ntot} cells, largest {max(sizes)}, "
                        f"smallest {min(sizes)}",
-           "precision": precision, "steps": steps, "warmup": warmup,
+           "precision": precision, "ordering": ordering, "steps": steps, "warmup": warmup,
            "batched": {"value": round(batched, 2), "unit": "Mcell-iters/s",
                        "ms_per_step": round(el * 1e3 / steps, 4), "kernel_set": ks,
                        "nonfinite_iter": int(bad)}}
@@ -280,7 +280,7 @@ def batch_sweep(ncases=64, steps=100, warmup=20, seq_steps=50, precision="fp32-m
     seq_time = 0.0
     seq_bad = 0
     for m, f in pairs:
-        with tdt.Solver(m, f, precision) as s:
+        with tdt.Solver(m, f, precision, ordering) as s:
             s.cold_start()
             s.iterate(warmup, history=False, check=False)
             s.synchronize()
@@ -295,7 +295,85 @@ def batch_sweep(ncases=64, steps=100, warmup=20, seq_steps=50, precision="fp32-m
     out["sequential"] = {"value": round(seq_rate, 2), "unit": "Mcell-iters/s",
                          "timed_steps_per_case": seq_steps, "nonfinite_iter": int(seq_bad)}
     out["speedup"] = round(batched / seq_rate, 3)
+    # the host's cores on the same cases (a process of its own: its workers fork,
+    # and a process that has used the GPU must not)
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-sweep", str(ncases),
+                        "--cpu-iters", "10"], capture_output=True, text=True, timeout=900)
+    if r.returncode == 0:
+        out["cpu_baseline"] = json.loads(r.stdout.strip().splitlines()[-1])
+        print(f"bench: the same cases on the host's cores {out['cpu_baseline']['value']} "
+              f"Mcell-iters/s", file=sys.stderr, flush=True)
+    else:
+        out["cpu_baseline"] = {"error": (r.stderr.strip().splitlines() or ["failed"])[-1]}
     return out
+
+
+_SWEEP = []   # the CPU sweep's cases, generated before its workers fork
+
+
+def _cpu_sweep_worker(idx, iters, barrier, out):
+    """One host core: its share of the sweep's cases in the CPU oracle, built
+    first, then (after every worker is ready) iterated, timed."""
+    if hasattr(os, "sched_getaffinity"):   # one allowed core per worker
+        allowed = sorted(os.sched_getaffinity(0))
+        os.sched_setaffinity(0, {allowed[idx[0] % len(allowed)]})
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    from oracle import Oracle
+    cases = idx[1]
+    orcs = []
+    for i in cases:
+        _, m, f = _SWEEP[i]
+        o = Oracle(m, f)
+        o.cold_start()
+        orcs.append((o, m.neles))
+    barrier.wait()
+    t0 = time.perf_counter()
+    for o, _ in orcs:
+        o.iterate(iters)
+    out.put((sum(n for _, n in orcs) * iters, time.perf_counter() - t0))
+
+
+def cpu_sweep(ncases: int, iters: int, cores: int):
+    """The batched sweep's cases on the host's cores: the CPU oracle (the
+    reference's arithmetic restated in C, FP64, one process per core), cases
+    spread over the cores by longest-processing-time (cell count), each core
+    iterating its cases `iters` times after all have set up.  Aggregate =
+    total cell-iterations / the slowest core's loop time."""
+    import multiprocessing as mp
+    global _SWEEP
+    _SWEEP = gridgen.sweep_cases(ncases)
+    sizes = [m.neles for _, m, _ in _SWEEP]
+    load = [0] * cores
+    share = [[] for _ in range(cores)]
+    for i in sorted(range(ncases), key=lambda i: -sizes[i]):
+        w = min(range(cores), key=lambda w: load[w])
+        share[w].append(i)
+        load[w] += sizes[i]
+    ctx = mp.get_context("fork")
+    barrier = ctx.Barrier(cores)
+    out = ctx.Queue()
+    procs = [ctx.Process(target=_cpu_sweep_worker, args=((w, share[w]), iters, barrier, out))
+             for w in range(cores)]
+    for p in procs:
+        p.start()
+    res = [out.get() for _ in procs]
+    for p in procs:
+        p.join()
+    work = sum(r[0] for r in res)
+    slowest = max(r[1] for r in res)
+    model = ""
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"value": round(work / slowest / 1e6, 3), "unit": "Mcell-iters/s", "cores": cores,
+            "kind": "port", "nproc": os.cpu_count(), "cpu": model,
+            "single_core": round(work / sum(r[1] for r in res) / 1e6, 4),
+            "sample": f"C oracle (FP64 restatement), {ncases} cases, {iters} iterations each, "
+                      f"LPT over {cores} processes; slowest core {slowest:.2f} s"}
 
 
 def copy_probe(mib: int) -> None:
@@ -326,6 +404,9 @@ def main():
     ap.add_argument("--batch", type=int, default=0, metavar="NCASES",
                     help="measure the batched parameter sweep of NCASES cases against the same "
                          "cases run one after another, print its JSON line and exit")
+    ap.add_argument("--cpu-sweep", type=int, default=0, metavar="NCASES", help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-iters", type=int, default=20, help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-cores", type=int, default=0, help=argparse.SUPPRESS)
     ap.add_argument("--no-sequential", action="store_true",
                     help="with --batch: skip the one-after-another runs")
     ap.add_argument("--no-large", action="store_true",
@@ -346,6 +427,18 @@ def main():
     if args.stream_probe:
         print(f"{stream_probe(args.stream_probe, args.stream_reps):.1f}", flush=True)
         return
+    if args.cpu_sweep:   # a process of its own that never touches the GPU (see batch_sweep)
+        avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+        cores = args.cpu_cores or min(16, avail or 1)
+        print(json.dumps(cpu_sweep(args.cpu_sweep, args.cpu_iters, cores)), flush=True)
+        return
+    if args.batch:
+        out = batch_sweep(args.batch, steps=args.steps if args.steps != 2000 else 100,
+                          warmup=min(args.warmup, 20), sequential=not args.no_sequential,
+                          ordering=args.ordering, precision=args.precision or "fp32-mixed")
+        print(json.dumps({"metric": "batched sweep, aggregate Mcell-iters/s on 1 MI355X",
+                          **out}), flush=True)
+        return
     if args.precision:
         mk_mesh, mk_flow = gridgen.WORKLOADS[args.workload]
         mesh, flow = mk_mesh(), mk_flow()
@@ -353,12 +446,6 @@ def main():
                                         args.ordering)
         print(f"bench: {args.precision} {mesh.neles * args.steps / el / 1e6:.1f} Mcell-iters/s "
               f"kernel set {ks} kernels(ms) {np.round(kt, 5).tolist()}", flush=True)
-        return
-    if args.batch:
-        out = batch_sweep(args.batch, steps=args.steps if args.steps != 2000 else 100,
-                          warmup=min(args.warmup, 20), sequential=not args.no_sequential)
-        print(json.dumps({"metric": "batched sweep, aggregate Mcell-iters/s on 1 MI355X",
-                          **out}), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); multi-GPU "
