@@ -9,83 +9,17 @@
 #include <unordered_map>
 #include <vector>
 
-#include "ctx.hpp"
+#include "api_internal.hpp"
 #include "reduce.hpp"
-#include "tdt.h"
 
 using tdt::Consts;
 using tdt::Dev;
+using namespace tdt::api;
 
-namespace {
+namespace tdt {
+namespace api {
 
 thread_local std::string g_create_error;
-
-#define CHECK_CTX(c)                   \
-    do {                               \
-        if (!(c)) return TDT_ERR_ARG;  \
-    } while (0)
-
-// Entry points of one case: a batch context runs its cases through the
-// tdt_batch_* calls only.
-#define CHECK_SINGLE(c)                                                         \
-    do {                                                                        \
-        if (!(c)) return TDT_ERR_ARG;                                           \
-        if ((c)->batch) {                                                       \
-            (c)->err = std::string(__func__) + ": not available on a batch context"; \
-            return TDT_ERR_UNSUPPORTED;                                         \
-        }                                                                       \
-    } while (0)
-
-int hip_fail(tdt_ctx *c, hipError_t e, const char *what) {
-    c->err = std::string(what) + ": " + hipGetErrorString(e);
-    return TDT_ERR_DEVICE;
-}
-
-#define HIPCALL(c, call, what)                                  \
-    do {                                                        \
-        hipError_t e__ = (call);                                \
-        if (e__ != hipSuccess) return hip_fail((c), e__, what); \
-    } while (0)
-
-template <class T>
-T *dev_alloc(tdt_ctx *c, size_t count, bool zero = true) {
-    void *p = nullptr;
-    size_t bytes = count * sizeof(T) + 256;
-    if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
-    if (zero && hipMemset(p, 0, bytes) != hipSuccess) {
-        hipFree(p);
-        return nullptr;
-    }
-    c->allocs.push_back(p);
-    return static_cast<T *>(p);
-}
-
-template <class R>
-Consts<R> make_consts(const tdt_params &P) {
-    // thermodynamics as in turbnstwod.f90:37-42; derived groupings keep the
-    // reference's evaluation order so they are bit-identical to it
-    double rr = 8314.0 / P.amol;
-    double cv = rr / (P.gama - 1.0);
-    double cp = cv * P.gama;
-    double rho0 = P.p0 / (rr * P.t0);
-    double s0 = P.p0 / std::pow(rho0, P.gama);
-    double tinf = P.t0 / (1.0 + 0.5 * (P.gama - 1.0) * P.aminf * P.aminf);
-    Consts<R> K{};
-    K.gama = (R)P.gama; K.cfl = (R)P.cfl; K.prandtl = (R)P.prandtl; K.prandtlt = (R)P.prandtlt;
-    K.t0 = (R)P.t0; K.pamb = (R)P.pamb; K.pinf = (R)P.pinf; K.deltabl = (R)P.deltabl;
-    K.rr = (R)rr; K.cv = (R)cv; K.cp = (R)cp; K.s0 = (R)s0; K.tinf = (R)tinf;
-    K.suth_sqrt_amol = (R)(tdt::kSuth * std::sqrt(P.amol));
-    K.inv_1mg = (R)(1.0 / (1.0 - P.gama));
-    K.sqcmu = (R)std::sqrt(2.0 * std::sqrt(tdt::kCmu));
-    K.bl_u = (R)(2.0 * P.aminf * std::sqrt(P.gama * rr * tinf));
-    K.gamma_rr = (R)(P.gama * rr);
-    K.tflame = (R)P.tflame; K.ancomb = (R)P.ancomb; K.acomb = (R)P.acomb;
-    K.rhoprop = (R)P.rhoprop;
-    K.gth_08 = (R)std::pow(tdt::kGth, 0.8);
-    K.local = P.local; K.npseudo = P.npseudo; K.iaxis = P.iaxis;
-    K.pamb_set = P.pamb >= 0.0;
-    return K;
-}
 
 void set_rinit(tdt_ctx *c, double rinit) {
     c->rinit = rinit;
@@ -95,6 +29,11 @@ void set_rinit(tdt_ctx *c, double rinit) {
     c->Kf.rtol = (float)rtol;
     c->Kf.rtol_rtred = (float)c->K.rtol_rtred;
 }
+
+}  // namespace api
+}  // namespace tdt
+
+namespace {
 
 // ---- named arrays --------------------------------------------------------------
 enum Kind { CELLGHOST, CELL, NODE, SCALAR };
@@ -153,13 +92,18 @@ int32_t kind_len(tdt_ctx *c, Kind k) {
     }
 }
 
-int sync(tdt_ctx *c) {
-    HIPCALL(c, hipStreamSynchronize(c->stream), "stream synchronize");
+int launched(tdt_ctx *c, hipError_t e, const char *what) {
+    if (e != hipSuccess) return hip_fail(c, e, what);
     return TDT_OK;
 }
 
-int launched(tdt_ctx *c, hipError_t e, const char *what) {
-    if (e != hipSuccess) return hip_fail(c, e, what);
+}  // namespace
+
+namespace tdt {
+namespace api {
+
+int sync(tdt_ctx *c) {
+    HIPCALL(c, hipStreamSynchronize(c->stream), "stream synchronize");
     return TDT_OK;
 }
 
@@ -173,9 +117,6 @@ bool params_ok(const tdt_params &P) {
     return true;
 }
 
-// Device side of a context whose mesh is prepared (c->pm): stream, uploads,
-// work arrays and the geometry (geometry.f90:55-149).  On failure the caller
-// destroys c.
 int init_device(tdt_ctx *c, const tdt_params *P) {
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
@@ -258,8 +199,6 @@ int init_device(tdt_ctx *c, const tdt_params *P) {
     return TDT_OK;
 }
 
-// Uniform initial state of a cold start (turbnstwod.f90:53-71) with the host's
-// libm, as the reference: q = C1..C5, *rinit = RINIT.
 void cold_state(const tdt_params &P, const Consts<double> &K, double q[5], double *rinit) {
     double gama = (double)K.gama;
     double am = P.amachinit;
@@ -279,143 +218,8 @@ void cold_state(const tdt_params &P, const Consts<double> &K, double q[5], doubl
     q[4] = rhoinit * *rinit;
 }
 
-// Batch: the cases' flow constants with their RTOL (set_rinit's arithmetic),
-// uploaded for the kernels.
-int batch_upload_consts(tdt_ctx *c) {
-    tdt::BatchInfo &bi = *c->batch;
-    for (int k = 0; k < bi.ncases; ++k) {
-        const double rtred = bi.params[k].rtred;
-        const double rtol = bi.rinit[k] * rtred;
-        bi.k64[k].rtol = rtol;
-        bi.k64[k].rtol_rtred = rtol * rtred;
-        bi.kf[k].rtol = (float)rtol;
-        bi.kf[k].rtol_rtred = (float)bi.k64[k].rtol_rtred;
-    }
-    HIPCALL(c, hipMemcpyAsync(bi.d_k64, bi.k64.data(), bi.k64.size() * sizeof(Consts<double>),
-                              hipMemcpyHostToDevice, c->stream), "batch constants");
-    HIPCALL(c, hipMemcpyAsync(bi.d_kf, bi.kf.data(), bi.kf.size() * sizeof(Consts<float>),
-                              hipMemcpyHostToDevice, c->stream), "batch constants");
-    return sync(c);
-}
-
-// Batch: case icase's cells (internal range) to or from host order.
-int batch_case_range(tdt_ctx *c, int icase, int *first, int *count) {
-    if (!c->batch) {
-        c->err = "not a batch context";
-        return TDT_ERR_ARG;
-    }
-    if (icase < 0 || icase >= c->batch->ncases) {
-        c->err = "case index out of range";
-        return TDT_ERR_ARG;
-    }
-    *first = c->batch->cell_off[icase];
-    *count = c->batch->ncell[icase];
-    return TDT_OK;
-}
-
-int adj_setup(tdt_ctx *c) {
-    if (c->adj) return TDT_OK;
-    if (!c->pm.g6cell.empty()) {
-        c->err = "residual adjoint: BC type 6 is not available";
-        return TDT_ERR_UNSUPPORTED;
-    }
-    const tdt::PreparedMesh &pm = c->pm;
-    const int n = pm.n;
-    // transposed stencil: entries (i, s) with member s of cell i equal to k, by i
-    std::vector<int> ptr(n + 1, 0);
-    auto member = [&](int i, int s) { return s == 0 ? i : pm.nc4[(size_t)4 * i + s - 1]; };
-    for (int i = 0; i < n; ++i)
-        for (int s = 0; s < 5; ++s) {
-            const int k = member(i, s);
-            if (k < n) ++ptr[k + 1];
-        }
-    for (int k = 0; k < n; ++k) ptr[k + 1] += ptr[k];
-    std::vector<int2> list(ptr[n]);
-    std::vector<int> fill(ptr.begin(), ptr.end() - 1);
-    for (int i = 0; i < n; ++i)
-        for (int s = 0; s < 5; ++s) {
-            const int k = member(i, s);
-            if (k < n) list[fill[k]++] = make_int2(i, s);
-        }
-    // node slots (the cell's nodes, then each face neighbour's): per-node lists of
-    // the (cell, slot) that hold it, by cell
-    const int m = pm.m;
-    auto slot_node = [&](int i, int sl) {
-        if (sl < 4) return pm.nod4[(size_t)4 * i + sl];
-        const int j = pm.nc4[(size_t)4 * i + (sl - 4) / 4];
-        return j < n ? pm.nod4[(size_t)4 * j + (sl - 4) % 4] : -1;
-    };
-    std::vector<int> nptr(m + 1, 0);
-    for (int i = 0; i < n; ++i)
-        for (int sl = 0; sl < 20; ++sl) {
-            const int v = slot_node(i, sl);
-            if (v >= 0) ++nptr[v + 1];
-        }
-    for (int v = 0; v < m; ++v) nptr[v + 1] += nptr[v];
-    std::vector<int2> nlist(nptr[m]);
-    std::vector<int> nfill(nptr.begin(), nptr.end() - 1);
-    for (int i = 0; i < n; ++i)
-        for (int sl = 0; sl < 20; ++sl) {
-            const int v = slot_node(i, sl);
-            if (v >= 0) nlist[nfill[v]++] = make_int2(i, sl);
-        }
-    auto a = std::make_unique<tdt::AdjBufs>();
-    a->Jx = dev_alloc<double>(c, (size_t)200 * n, false);
-    a->gxy = dev_alloc<double>(c, (size_t)2 * m);
-    a->nptr = dev_alloc<int>(c, m + 1);
-    a->nlist = dev_alloc<int2>(c, nlist.size());
-    a->q = dev_alloc<double>(c, (size_t)5 * n);
-    a->v = dev_alloc<double>(c, (size_t)5 * n);
-    a->g = dev_alloc<double>(c, (size_t)5 * n);
-    a->r = dev_alloc<double>(c, (size_t)5 * n);
-    a->J = dev_alloc<double>(c, (size_t)125 * n, false);
-    a->Jp = dev_alloc<double>(c, (size_t)10 * n, false);
-    a->part = dev_alloc<double>(c, (size_t)2 * tdt::nblocks(n));
-    a->gp = dev_alloc<double>(c, 2);
-    a->tptr = dev_alloc<int>(c, n + 1);
-    a->tlist = dev_alloc<int2>(c, list.size());
-    bool ok = a->q && a->v && a->g && a->r && a->J && a->Jp && a->part && a->gp && a->tptr &&
-              a->tlist && a->Jx && a->gxy && a->nptr && a->nlist;
-    ok = ok && hipMemcpy(a->nptr, nptr.data(), nptr.size() * sizeof(int),
-                         hipMemcpyHostToDevice) == hipSuccess;
-    ok = ok && hipMemcpy(a->nlist, nlist.data(), nlist.size() * sizeof(int2),
-                         hipMemcpyHostToDevice) == hipSuccess;
-    ok = ok && hipMemcpy(a->tptr, ptr.data(), ptr.size() * sizeof(int),
-                         hipMemcpyHostToDevice) == hipSuccess;
-    ok = ok && hipMemcpy(a->tlist, list.data(), list.size() * sizeof(int2),
-                         hipMemcpyHostToDevice) == hipSuccess;
-    if (!ok) {
-        c->err = "residual adjoint: device allocation failed";
-        return TDT_ERR_DEVICE;
-    }
-    c->adj = a.release();
-    return TDT_OK;
-}
-
-// [5][neles] in original ids <-> [5][n] internal order on the device
-int upload_cells(tdt_ctx *c, const double *src, double *dst) {
-    const size_t n = c->pm.n;
-    std::vector<double> h(5 * n);
-    for (int q = 0; q < 5; ++q)
-        for (size_t i = 0; i < n; ++i) h[q * n + i] = src[q * n + c->pm.cell_new2old[i]];
-    HIPCALL(c, hipMemcpyAsync(dst, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice,
-                              c->stream), "upload");
-    return sync(c);
-}
-
-int download_cells(tdt_ctx *c, const double *src, double *dst) {
-    const size_t n = c->pm.n;
-    std::vector<double> h(5 * n);
-    HIPCALL(c, hipMemcpyAsync(h.data(), src, h.size() * sizeof(double), hipMemcpyDeviceToHost,
-                              c->stream), "download");
-    int rc = sync(c);
-    if (rc) return rc;
-    for (int q = 0; q < 5; ++q)
-        for (size_t i = 0; i < n; ++i) dst[q * n + c->pm.cell_new2old[i]] = h[q * n + i];
-    return TDT_OK;
-}
-
-}  // namespace
+}  // namespace api
+}  // namespace tdt
 
 extern "C" {
 
@@ -664,366 +468,6 @@ int tdt_kernel_set(tdt_ctx *c, int32_t *set) {
 int tdt_synchronize(tdt_ctx *c) {
     CHECK_CTX(c);
     return sync(c);
-}
-
-// ---- residual and its adjoint ------------------------------------------------------
-
-int tdt_set_params(tdt_ctx *c, const tdt_params *P) {
-    CHECK_SINGLE(c);
-    if (!P) return TDT_ERR_ARG;
-    if (!params_ok(*P)) {
-        c->err = g_create_error;
-        return TDT_ERR_ARG;
-    }
-    if (P->iaxis != c->params.iaxis) {
-        c->err = "tdt_set_params: IAXIS fixes the geometry and cannot change";
-        return TDT_ERR_UNSUPPORTED;
-    }
-    int rc = sync(c);
-    if (rc) return rc;
-    c->params = *P;
-    c->params_rtred = P->rtred;
-    c->K = make_consts<double>(*P);
-    c->Kf = make_consts<float>(*P);
-    set_rinit(c, c->rinit);
-    tdt::fused_free(c);   // its graphs hold the old constants; rebuilt on the next iterate
-    return TDT_OK;
-}
-
-int tdt_set_nodes(tdt_ctx *c, const double *x_mm, const double *y_mm) {
-    CHECK_SINGLE(c);
-    if (!x_mm || !y_mm) return TDT_ERR_ARG;
-    const tdt::PreparedMesh &pm = c->pm;
-    std::vector<double> h((size_t)2 * pm.m);
-    for (int v = 0; v < pm.m; ++v) {
-        // X(N) = X(N) * MM_TO_M  (geometry.f90:35-36)
-        h[v] = x_mm[pm.node_new2old[v]] * 0.001;
-        h[(size_t)pm.m + v] = y_mm[pm.node_new2old[v]] * 0.001;
-    }
-    int rc = sync(c);
-    if (rc) return rc;
-    HIPCALL(c, hipMemcpyAsync(const_cast<double *>(c->D.xn), h.data(), pm.m * sizeof(double),
-                              hipMemcpyHostToDevice, c->stream), "node upload");
-    HIPCALL(c, hipMemcpyAsync(const_cast<double *>(c->D.yn), h.data() + pm.m,
-                              pm.m * sizeof(double), hipMemcpyHostToDevice, c->stream),
-            "node upload");
-    HIPCALL(c, tdt::launch_geometry(c, c->params.iaxis), "geometry");
-    ++c->geom_version;
-    return sync(c);
-}
-
-int tdt_residual_eval(tdt_ctx *c, const double *cst, double *r) {
-    CHECK_SINGLE(c);
-    if (!cst || !r) return TDT_ERR_ARG;
-    int rc = adj_setup(c);
-    if (!rc) rc = upload_cells(c, cst, c->adj->q);
-    if (rc) return rc;
-    HIPCALL(c, tdt::launch_res_eval(c, c->adj->q, c->adj->r), "residual");
-    return download_cells(c, c->adj->r, r);
-}
-
-int tdt_residual_vjp(tdt_ctx *c, const double *cst, const double *v, double *gq,
-                     double *gparam, double *gxy) {
-    CHECK_SINGLE(c);
-    if (!cst || !v || !gq) return TDT_ERR_ARG;
-    int rc = adj_setup(c);
-    if (!rc) rc = upload_cells(c, cst, c->adj->q);
-    if (!rc) rc = upload_cells(c, v, c->adj->v);
-    if (rc) return rc;
-    tdt::AdjBufs &a = *c->adj;
-    HIPCALL(c, tdt::launch_res_vjp(c, a.q, a.v, a.tptr, a.tlist, a.J, a.Jp, a.part, a.g, a.gp,
-                                   a.Jx, a.nptr, a.nlist, gxy ? a.gxy : nullptr),
-            "residual adjoint");
-    rc = download_cells(c, a.g, gq);
-    if (rc) return rc;
-    if (gparam) {
-        double gp[2] = {0.0, 0.0};
-        HIPCALL(c, hipMemcpyAsync(gp, a.gp, sizeof gp, hipMemcpyDeviceToHost, c->stream),
-                "download");
-        rc = sync(c);
-        if (rc) return rc;
-        gparam[0] = gp[0];
-        gparam[1] = gp[1];
-        gparam[2] = 0.0;   // R does not depend on AMACHINIT at a given state
-    }
-    if (gxy) {
-        // per node, d/d(x, y) in metres -> grid.in millimetres (X = X_MM * 0.001)
-        const int m = c->pm.m;
-        std::vector<double> h((size_t)2 * m);
-        HIPCALL(c, hipMemcpyAsync(h.data(), a.gxy, h.size() * sizeof(double),
-                                  hipMemcpyDeviceToHost, c->stream), "download");
-        rc = sync(c);
-        if (rc) return rc;
-        for (int v2 = 0; v2 < m; ++v2) {
-            const int o = c->pm.node_new2old[v2];
-            gxy[o] = h[v2] * 0.001;
-            gxy[(size_t)m + o] = h[(size_t)m + v2] * 0.001;
-        }
-    }
-    return TDT_OK;
-}
-
-// ---- batched cases ----------------------------------------------------------------
-
-int tdt_batch_create(tdt_ctx **out, int32_t ncases, const tdt_mesh *meshes,
-                     const tdt_params *params, int32_t precision, int32_t ordering) {
-    g_create_error.clear();
-    if (!out || !meshes || !params || ncases < 1) {
-        g_create_error = "null argument or no cases";
-        return TDT_ERR_ARG;
-    }
-    *out = nullptr;
-    if (precision != TDT_FP64 && precision != TDT_FP32_MIXED) {
-        g_create_error = "unknown precision";
-        return TDT_ERR_ARG;
-    }
-    for (int k = 0; k < ncases; ++k) {
-        if (!params_ok(params[k])) {
-            g_create_error = "case " + std::to_string(k) + ": " + g_create_error;
-            return TDT_ERR_ARG;
-        }
-        // per case: P0, T0, AMACHINIT, PAMB, RTRED and the BC type-7 inputs; the
-        // gas and model constants are the batch's
-        const tdt_params &a = params[k], &b = params[0];
-        if (a.local != 1 || a.iaxis != b.iaxis || a.gama != b.gama || a.amol != b.amol ||
-            a.cfl != b.cfl || a.prandtl != b.prandtl || a.prandtlt != b.prandtlt ||
-            a.npseudo != b.npseudo) {
-            g_create_error = "case " + std::to_string(k) +
-                             ": a batch needs LOCAL = 1 and the same IAXIS, GAMA, AMOL, CFL, "
-                             "PRANDTL, PRANDTLT and NPSEUDO in every case";
-            return TDT_ERR_UNSUPPORTED;
-        }
-    }
-    // each case prepared alone: its validation and the cell order it would get alone
-    auto bi = std::make_unique<tdt::BatchInfo>();
-    bi->ncases = ncases;
-    std::vector<int> order;
-    size_t N = 0, G = 0, M = 0;
-    for (int k = 0; k < ncases; ++k) {
-        const tdt_mesh &ms = meshes[k];
-        tdt::PreparedMesh pk;
-        int code = TDT_OK;
-        std::string msg = tdt::prepare_mesh(ms, ordering, pk, &code);
-        if (!msg.empty()) {
-            g_create_error = "case " + std::to_string(k) + ": " + msg;
-            return code;
-        }
-        if (!pk.g6cell.empty()) {
-            g_create_error = "case " + std::to_string(k) + ": BC type 6 is not available in a batch";
-            return TDT_ERR_UNSUPPORTED;
-        }
-        if (N + ms.neles + G + ms.nghosts > ((size_t)1 << 28)) {   // the fused path's id range
-            g_create_error = "batch: too many cells and ghosts";
-            return TDT_ERR_UNSUPPORTED;
-        }
-        bi->cell_off.push_back((int)N);
-        bi->ncell.push_back(ms.neles);
-        bi->ghost_off.push_back((int)G);
-        bi->nghost.push_back(ms.nghosts);
-        bi->orig_cell_off.push_back((int)N);
-        for (int q : pk.cell_new2old) order.push_back((int)N + q);
-        N += ms.neles;
-        G += ms.nghosts;
-        M += ms.nods;
-    }
-    // the concatenated grid: cells, then ghosts, of case 0, 1, ... (grid.in ids offset)
-    std::vector<double> x(M), y(M);
-    std::vector<int32_t> nod(4 * N), nc(4 * N), par(G), gh(G), ty(G), sd(G);
-    size_t moff = 0;
-    for (int k = 0; k < ncases; ++k) {
-        const tdt_mesh &ms = meshes[k];
-        const int n = ms.neles, noff = bi->cell_off[k], goff = bi->ghost_off[k];
-        std::unordered_map<int, int> gpos;
-        for (int j = 0; j < ms.nghosts; ++j) gpos[ms.ghost[j]] = j;
-        for (int v = 0; v < ms.nods; ++v) {
-            x[moff + v] = ms.x_mm[v];
-            y[moff + v] = ms.y_mm[v];
-        }
-        for (int f = 0; f < 4; ++f)
-            for (int i = 0; i < n; ++i) {
-                const size_t s = (size_t)f * n + i, d = (size_t)f * N + noff + i;
-                nod[d] = (int32_t)(moff + ms.nod[s]);
-                const int v = ms.nc[s];
-                nc[d] = v >= 1 && v <= n ? noff + v
-                        : v == -1        ? -1
-                                         : (int32_t)(N + goff + gpos.at(v) + 1);
-            }
-        for (int j = 0; j < ms.nghosts; ++j) {
-            par[goff + j] = noff + ms.parent[j];
-            gh[goff + j] = (int32_t)(N + goff + j + 1);
-            ty[goff + j] = ms.type[j];
-            sd[goff + j] = ms.side[j];
-        }
-        moff += ms.nods;
-    }
-    tdt_mesh all{};
-    all.nods = (int32_t)M;
-    all.neles = (int32_t)N;
-    all.nghosts = (int32_t)G;
-    all.x_mm = x.data();
-    all.y_mm = y.data();
-    all.nod = nod.data();
-    all.nc = nc.data();
-    all.parent = par.data();
-    all.ghost = gh.data();
-    all.type = ty.data();
-    all.side = sd.data();
-    tdt_ctx *c = new tdt_ctx();
-    c->precision = precision;
-    int code = TDT_OK;
-    std::string msg = tdt::prepare_mesh(all, TDT_ORDER_INPUT, c->pm, &code, &order);
-    if (!msg.empty()) {
-        g_create_error = "batch: " + msg;
-        delete c;
-        return code;
-    }
-    code = init_device(c, &params[0]);
-    if (code != TDT_OK) {
-        tdt_destroy(c);
-        return code;
-    }
-    set_rinit(c, 0.0);
-    // cell blocks: each holds cells of one case
-    for (int k = 0; k < ncases; ++k) {
-        const int first = (int)bi->cblk.size(), n = bi->ncell[k];
-        for (int b = 0; b * tdt::kBlock < n; ++b)
-            bi->cblk.push_back(make_int4(bi->cell_off[k] + b * tdt::kBlock,
-                                         std::min(tdt::kBlock, n - b * tdt::kBlock), k, 0));
-        bi->ccase.push_back(make_int4(first, (int)bi->cblk.size() - first, n, 0));
-    }
-    bi->nb_cells = (int)bi->cblk.size();
-    while (bi->cblk.size() % 8) bi->cblk.push_back(make_int4(0, 0, 0, 0));
-    // node blocks: a node belongs to the case of the first cell that touches it
-    // (nodes no cell touches come last; they are given case 0)
-    const tdt::PreparedMesh &pm = c->pm;
-    std::vector<int> ncase(pm.m, -1);
-    for (int k = 0; k < ncases; ++k)
-        for (int i = bi->cell_off[k]; i < bi->cell_off[k] + bi->ncell[k]; ++i)
-            for (int f = 0; f < 4; ++f) {
-                int &v = ncase[pm.nod4[(size_t)4 * i + f]];
-                if (v < 0) v = k;
-            }
-    for (int v = 0; v < pm.m;) {
-        const int k = ncase[v] < 0 ? 0 : ncase[v];
-        int e = v + 1;
-        while (e < pm.m && e - v < tdt::kBlock && (ncase[e] < 0 ? 0 : ncase[e]) == k) ++e;
-        bi->nblk.push_back(make_int4(v, e - v, k, 0));
-        v = e;
-    }
-    bi->nb_nodes = (int)bi->nblk.size();
-    while (bi->nblk.size() % 8) bi->nblk.push_back(make_int4(0, 0, 0, 0));
-    for (int k = 0; k < ncases; ++k) {
-        bi->params.push_back(params[k]);
-        bi->k64.push_back(make_consts<double>(params[k]));
-        bi->kf.push_back(make_consts<float>(params[k]));
-    }
-    bi->rinit.assign(ncases, 0.0);
-    bi->d_k64 = dev_alloc<Consts<double>>(c, ncases);
-    bi->d_kf = dev_alloc<Consts<float>>(c, ncases);
-    bi->d_cblk = dev_alloc<int4>(c, bi->cblk.size());
-    bi->d_nblk = dev_alloc<int4>(c, bi->nblk.size());
-    bi->d_ccase = dev_alloc<int4>(c, ncases);
-    bool ok = bi->d_k64 && bi->d_kf && bi->d_cblk && bi->d_nblk && bi->d_ccase;
-    auto up = [&](void *dst, const void *src, size_t bytes) {
-        if (ok) ok = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
-    };
-    up(bi->d_cblk, bi->cblk.data(), bi->cblk.size() * sizeof(int4));
-    up(bi->d_nblk, bi->nblk.data(), bi->nblk.size() * sizeof(int4));
-    up(bi->d_ccase, bi->ccase.data(), bi->ccase.size() * sizeof(int4));
-    c->batch = bi.release();
-    if (!ok || batch_upload_consts(c) != TDT_OK) {
-        g_create_error = "batch: device allocation or upload failed";
-        tdt_destroy(c);
-        return TDT_ERR_DEVICE;
-    }
-    *out = c;
-    return TDT_OK;
-}
-
-int tdt_batch_cold_start(tdt_ctx *c, double *rinit) {
-    CHECK_CTX(c);
-    if (!c->batch) {
-        c->err = "not a batch context";
-        return TDT_ERR_ARG;
-    }
-    tdt::BatchInfo &bi = *c->batch;
-    const size_t n = c->pm.n, nt = c->D.nt;
-    std::vector<double> h(5 * n);
-    for (int k = 0; k < bi.ncases; ++k) {
-        double q[5];
-        cold_state(bi.params[k], bi.k64[k], q, &bi.rinit[k]);
-        for (int m = 0; m < 5; ++m)
-            std::fill(h.begin() + m * n + bi.cell_off[k],
-                      h.begin() + m * n + bi.cell_off[k] + bi.ncell[k], q[m]);
-        if (rinit) rinit[k] = bi.rinit[k];
-    }
-    for (int m = 0; m < 5; ++m) {
-        HIPCALL(c, hipMemcpyAsync(c->D.c + m * nt, h.data() + m * n, n * sizeof(double),
-                                  hipMemcpyHostToDevice, c->stream), "batch cold start");
-        HIPCALL(c, hipMemcpyAsync(c->D.cold + m * n, h.data() + m * n, n * sizeof(double),
-                                  hipMemcpyHostToDevice, c->stream), "batch cold start");
-    }
-    return batch_upload_consts(c);
-}
-
-int tdt_batch_set_state(tdt_ctx *c, int32_t icase, const double *cst, double rinit) {
-    CHECK_CTX(c);
-    int first = 0, cnt = 0;
-    int rc = batch_case_range(c, icase, &first, &cnt);
-    if (rc) return rc;
-    if (!cst) return TDT_ERR_ARG;
-    const tdt::BatchInfo &bi = *c->batch;
-    const size_t nt = c->D.nt, n = c->pm.n;
-    std::vector<double> h(5 * (size_t)cnt);
-    for (int i = 0; i < cnt; ++i) {
-        const int old = c->pm.cell_new2old[first + i] - bi.orig_cell_off[icase];
-        for (int m = 0; m < 5; ++m) h[(size_t)m * cnt + i] = cst[(size_t)m * cnt + old];
-    }
-    for (int m = 0; m < 5; ++m) {
-        HIPCALL(c, hipMemcpyAsync(c->D.c + m * nt + first, h.data() + (size_t)m * cnt,
-                                  cnt * sizeof(double), hipMemcpyHostToDevice, c->stream),
-                "batch state upload");
-        HIPCALL(c, hipMemcpyAsync(c->D.cold + m * n + first, h.data() + (size_t)m * cnt,
-                                  cnt * sizeof(double), hipMemcpyHostToDevice, c->stream),
-                "batch state upload");
-    }
-    c->batch->rinit[icase] = rinit;
-    return batch_upload_consts(c);
-}
-
-int tdt_batch_get_state(tdt_ctx *c, int32_t icase, double *cst) {
-    CHECK_CTX(c);
-    int first = 0, cnt = 0;
-    int rc = batch_case_range(c, icase, &first, &cnt);
-    if (rc) return rc;
-    if (!cst) return TDT_ERR_ARG;
-    const size_t nt = c->D.nt;
-    std::vector<double> h(5 * (size_t)cnt);
-    for (int m = 0; m < 5; ++m)
-        HIPCALL(c, hipMemcpyAsync(h.data() + (size_t)m * cnt, c->D.c + m * nt + first,
-                                  cnt * sizeof(double), hipMemcpyDeviceToHost, c->stream),
-                "batch state download");
-    rc = sync(c);
-    if (rc) return rc;
-    const tdt::BatchInfo &bi = *c->batch;
-    for (int i = 0; i < cnt; ++i) {
-        const int old = c->pm.cell_new2old[first + i] - bi.orig_cell_off[icase];
-        for (int m = 0; m < 5; ++m) cst[(size_t)m * cnt + old] = h[(size_t)m * cnt + i];
-    }
-    return TDT_OK;
-}
-
-int tdt_batch_iterate(tdt_ctx *c, int32_t niter, double *l2, double *linf,
-                      int32_t *first_nonfinite) {
-    CHECK_CTX(c);
-    if (!c->batch) {
-        c->err = "not a batch context";
-        return TDT_ERR_ARG;
-    }
-    if (niter < 0) return TDT_ERR_ARG;
-    if (first_nonfinite) *first_nonfinite = 0;
-    if (niter == 0) return TDT_OK;
-    return tdt::fused_iterate(c, niter, l2, linf, first_nonfinite);
 }
 
 }  // extern "C"
