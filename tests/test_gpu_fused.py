@@ -278,3 +278,65 @@ def test_fp32_mixed_error_at_duct_run_length(monkeypatch):
     err = analysis.compare(mesh, flow, states["fp32-mixed"], states["fp64"])
     print("fp32-mixed vs fp64 after", flow.niter, "iterations:", err)
     assert err["p"] <= 1e-5 and err["rho"] <= 1e-5, err
+
+
+@pytest.fixture(scope="module")
+def full_ramp():
+    """The 2049x513 compression ramp (1,051,137 cells), the roofline workload."""
+    from tdt import gridgen
+    mk_mesh, mk_flow = gridgen.WORKLOADS["ramp_2049x513"]
+    return mk_mesh(), mk_flow()
+
+
+def test_full_size_ramp_fused_equals_routine_path(full_ramp, monkeypatch):
+    """At the full 1.05 M-cell size (boundary-last cell order), two fused
+    iterations equal two iterations of the routine entry points bit for bit,
+    norms included (the reference's own fixtures stop at 50 k cells)."""
+    import tdt
+    mesh, flow = full_ramp
+    monkeypatch.setenv("TDT_KERNELS", "split")
+    with tdt.Solver(mesh, flow, "fp64") as a, tdt.Solver(mesh, flow, "fp64") as b:
+        assert a.ordering_used == 3        # boundary-last at this size
+        a.cold_start()
+        b.cold_start()
+        hist = []
+        for _ in range(2):
+            a.timestep()
+            a.dissipation()
+            for st in range(4):
+                a.primitives()
+                a.apply_boundary()
+                if st == 0:
+                    a.viscous()
+                a.source_terms()
+                a.residual()
+                a.update(ALPHAS[st])
+            hist.append(a.monitor())
+        l2, linf, bad = b.iterate(2)
+        assert bad == 0
+        assert np.array_equal(a.get_state(), b.get_state())
+        assert np.array_equal(np.array([h[0] for h in hist]), l2)
+        assert np.array_equal(np.array([h[1] for h in hist]), linf)
+
+
+def test_full_size_ramp_orderings_and_precisions(full_ramp):
+    """Full size: input order and boundary-last order give bitwise identical
+    FP64 states; FP32-mixed stays within 1e-5 of FP64 in rho and p after 20
+    iterations of the Mach 2.5 transient (3.7e-6 in rho measured); the
+    kernel-set choice is one of the candidates."""
+    import tdt
+    mesh, flow = full_ramp
+    states = {}
+    for prec, ordering in (("fp64", "input"), ("fp64", "auto"), ("fp32-mixed", "auto")):
+        with tdt.Solver(mesh, flow, prec, ordering) as s:
+            s.cold_start()
+            _, _, bad = s.iterate(20)
+            assert bad == 0
+            assert s.kernel_set() != ""
+            states[(prec, ordering)] = s.get_state()
+    assert np.array_equal(states[("fp64", "input")], states[("fp64", "auto")])
+    a = tdt.primitives_from_state(states[("fp32-mixed", "auto")], flow)
+    b = tdt.primitives_from_state(states[("fp64", "auto")], flow)
+    for k in ("rho", "p"):
+        err = np.linalg.norm(a[k] - b[k]) / np.linalg.norm(b[k])
+        assert err < 1e-5, (k, err)
