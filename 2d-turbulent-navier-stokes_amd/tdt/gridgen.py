@@ -278,6 +278,15 @@ WORKLOADS = {
                       lambda: bl_inlet_params(flow_channel())),
     "duct_385x129": (lambda seed=DEFAULT_SEED: duct(seed=seed), flow_duct),
     "ramp_2049x513": (lambda seed=DEFAULT_SEED: ramp(seed=seed), flow_ramp),
+    # the same ramp with the 1/7-power boundary-layer inlet (type 7: p, T and u
+    # fixed at PINF, TINF and the Mach 2.5 profile) on the left instead of
+    # extrapolation (type 3).  Extrapolating every variable at a supersonic
+    # inflow fixes none of them, and the type-3 ramp's inflow column drifts until
+    # the run goes non-finite (iteration 3709 in FP64, in this library and in the
+    # oracle alike); the type-7 ramp stays finite over its 20000 iterations.
+    "ramp_2049x513_t7": (lambda seed=DEFAULT_SEED: ramp(seed=seed, bc={"left": 7, "right": 3,
+                                                                        "bottom": 5, "top": 3}),
+                         lambda: bl_inlet_params(flow_ramp())),
     # the adjoint configuration's grid: the duct at 193x65 with 8 Hicks-Henne bumps
     # (amplitudes 0 here; a design study sets them).  Its nodes move as far in x as
     # the 385-column duct's (1 % of that spacing): 1 % of the coarser spacing moves

@@ -33,7 +33,10 @@ from tdt import analysis, caseio, gridgen  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 DESCRIBE = {"channel_97x49": "2D turbulent channel / flat plate",
             "duct_385x129": "axisymmetric converging-diverging duct",
-            "ramp_2049x513": "2D supersonic 15-degree compression ramp"}
+            "ramp_2049x513": "2D supersonic 15-degree compression ramp",
+            "ramp_2049x513_t7": "2D supersonic 15-degree compression ramp, type-7 "
+                                "(fixed p, T, u) inflow",
+            "duct_193x65": "axisymmetric converging-diverging duct with 8 Hicks-Henne bumps"}
 KERNELS = ("k_pre", "k_dv", "k_stage1", "k_stage2", "k_stage3", "k_stage4")
 
 
@@ -135,41 +138,57 @@ def converged_comparison(mesh, flow, iters):
     for prec in ("fp64", "fp32-mixed"):
         with tdt.Solver(mesh, flow, prec) as s:
             s.cold_start()
-            l2, _, bad = s.iterate(iters)
+            l2, _, bad = s.iterate(iters, check=False)
             states[prec] = s.get_state()
             drop = l2[-1] / np.maximum(l2[:10].max(axis=0), 1e-300)
             out[f"{prec}_l2_drop"] = [float(f"{v:.3e}") for v in drop]
             out[f"{prec}_nonfinite_iter"] = int(bad)
-    err = analysis.compare(mesh, flow, states["fp32-mixed"], states["fp64"])
-    out["rel_l2_vs_fp64"] = {k: float(f"{v:.3e}") for k, v in err.items()}
+    if all(np.isfinite(st).all() for st in states.values()):
+        err = analysis.compare(mesh, flow, states["fp32-mixed"], states["fp64"])
+        out["rel_l2_vs_fp64"] = {k: float(f"{v:.3e}") for k, v in err.items()}
     return out
 
 
+REF_BUILDS = (("turbnstwod", 60000), ("turbnstwod_big", 1060000))   # (executable, its NN)
+
+
 def cpu_baseline(mesh, flow, target_s: float = 15.0):
-    """The reference executable (built from its sources by oracle/build_ref.sh)
-    on this host, one core; loop time = t(NITER=k) - t(NITER=0)."""
-    ref = os.path.join(REPO, "oracle", "_ref", "turbnstwod")
-    kind = "reference"
-    if not os.path.exists(ref) or mesh.neles + mesh.nghosts > 60000:
+    """The reference executable (built from its sources by oracle/build_ref.sh,
+    the smallest build whose NN holds the grid's cells + ghosts) on this host,
+    one core.  The case is written once; loop time = t(NITER=k) - t(NITER=0),
+    so reading the case and writing the results cancel out."""
+    ref = None
+    for name, nn in REF_BUILDS:
+        path = os.path.join(REPO, "oracle", "_ref", name)
+        if mesh.neles + mesh.nghosts <= nn and os.path.exists(path):
+            ref = path
+            break
+    if ref is None:
         return cpu_baseline_port(mesh, flow, target_s)
     work = tempfile.mkdtemp(prefix="tdt_cpu_")
     try:
+        caseio.write_case(work, caseio.Flow(**{**flow.__dict__, "niter": 0, "istart": 0}), mesh)
+
         def run(niter):
-            f = caseio.Flow(**{**flow.__dict__, "niter": niter, "istart": 0})
-            caseio.write_case(work, f, mesh)
+            caseio.write_flow(os.path.join(work, "flow.in"),
+                              caseio.Flow(**{**flow.__dict__, "niter": niter, "istart": 0}))
             t0 = time.perf_counter()
             subprocess.run(["taskset", "-c", "0", ref] if shutil.which("taskset") else [ref],
                            cwd=work, stdout=subprocess.DEVNULL, check=True)
             return time.perf_counter() - t0
-        t_probe = run(5) - run(0)
-        per_iter = max(t_probe / 5, 1e-6)
-        k = int(max(10, min(2000, target_s / per_iter)))
-        t_k, t_0 = run(k), run(0)
-        loop = max(t_k - t_0, 1e-9)
+        t_0 = run(0)
+        k = 5 if mesh.neles < 200000 else 8     # about 1 s per iteration at 1 M cells
+        loop = run(k) - t_0
+        if loop < 0.5 * target_s:
+            k = int(max(k + 1, min(2000, k * target_s / max(loop, 1e-6))))
+            loop = run(k) - t_0
+        loop = max(loop, 1e-9)
         value = mesh.neles * k / loop / 1e6
-        return {"value": round(value, 4), "unit": "Mcell-iters/s", "cores": 1, "kind": kind,
-                "sample": f"reference turbnstwod (FP64, amdflang -O3), same grid and flow.in, "
-                          f"{k} iterations, loop time = t(NITER={k}) - t(NITER=0) = {loop:.2f} s"}
+        return {"value": round(value, 4), "unit": "Mcell-iters/s", "cores": 1,
+                "kind": "reference",
+                "sample": f"reference {os.path.basename(ref)} (FP64, amdflang -O3), same grid "
+                          f"and flow.in, {k} iterations, loop time = t(NITER={k}) - "
+                          f"t(NITER=0) = {loop:.2f} s"}
     finally:
         shutil.rmtree(work, ignore_errors=True)
 
@@ -222,26 +241,68 @@ def stream_probe(mib: int, reps: int) -> float:
     return 2 * n * 8 / sec / 1e9
 
 
-def large_case(stream_gbs, steps=200, warmup=50):
-    """The 2049x513 compression ramp (1.05 M cells: enough wavefronts to fill
-    the GPU), FP64 and FP32-mixed: throughput and the roofline of the dominant
-    kernel of each, for the HBM-bound regime the 385x129 duct cannot reach."""
-    mk_mesh, mk_flow = gridgen.WORKLOADS["ramp_2049x513"]
+def other_case(workload, stream_gbs, steps, warmup, converge, cpu):
+    """A second workload beside the main one, FP64 and FP32-mixed: throughput,
+    the roofline of each one's dominant kernel, the converged FP32-mixed vs FP64
+    comparison over `converge` iterations (0 skips it) and, with `cpu`, the
+    reference on one host core."""
+    mk_mesh, mk_flow = gridgen.WORKLOADS[workload]
     mesh, flow = mk_mesh(), mk_flow()
-    out = {"workload": f"ramp_2049x513: {DESCRIBE['ramp_2049x513']}, {mesh.neles} cells",
+    out = {"workload": f"{workload}: {DESCRIBE[workload]}, {mesh.neles} cells + "
+                       f"{mesh.nghosts} ghosts, IAXIS={flow.iaxis}, P0={flow.p0}, T0={flow.t0}, "
+                       f"M={flow.amachinit}",
            "steps": steps, "warmup": warmup}
+    rate = {}
     for prec in ("fp64", "fp32-mixed"):
         el, _, _, bad, kt, ks = run_gpu(mesh, flow, prec, steps, warmup)
         v = mesh.neles * steps / el / 1e6
-        roof = roofline("ramp_2049x513", prec, mesh, flow, kt, ks, el, steps)
+        rate[prec] = v
+        roof = roofline(workload, prec, mesh, flow, kt, ks, el, steps)
         if stream_gbs:
             roof["frac_of_stream"] = round(roof["achieved"] / stream_gbs, 4)
-        out[prec] = {"value": round(v, 2), "unit": "Mcell-iters/s", "kernel_set": ks,
+        out[prec] = {"value": round(v, 2), "unit": "Mcell-iters/s",
+                     "ms_per_step": round(el * 1e3 / steps, 5), "kernel_set": ks,
                      "kernel_ms": [round(float(x), 5) for x in kt], "nonfinite_iter": int(bad),
                      "roofline": roof}
-        print(f"bench: ramp {prec} {v:.1f} Mcell-iters/s kernel set {ks} kernels(ms) "
+        print(f"bench: {workload} {prec} {v:.1f} Mcell-iters/s kernel set {ks} kernels(ms) "
               f"{np.round(kt, 5).tolist()}", file=sys.stderr, flush=True)
+    out["fp32-mixed"]["speedup_vs_fp64"] = round(rate["fp32-mixed"] / rate["fp64"], 3)
+    if converge > 0:
+        out["converged"] = converged_comparison(mesh, flow, converge)
+        print(f"bench: {workload} converged comparison {out['converged']}", file=sys.stderr,
+              flush=True)
+    if cpu:
+        out["cpu_baseline"] = cpu_baseline(mesh, flow)
+        print(f"bench: {workload} CPU {out['cpu_baseline']}", file=sys.stderr, flush=True)
     return out
+
+
+def large_case(stream_gbs, steps=200, warmup=50, converge=20000, cpu=True):
+    """The 2049x513 compression ramp (1.05 M cells: enough wavefronts to fill
+    the GPU), for the HBM-bound regime the 385x129 duct cannot reach.  With its
+    type-3 (extrapolated) Mach 2.5 inflow the case goes non-finite within 4000
+    iterations, so `inflow_type3` records where each precision does, and the
+    FP32-mixed vs FP64 comparison runs the case's 20000 iterations on the same
+    ramp with the type-7 (fixed p, T, u) inflow."""
+    out = other_case("ramp_2049x513", stream_gbs, steps, warmup, 0, cpu)
+    if converge > 0:
+        mk_mesh, mk_flow = gridgen.WORKLOADS["ramp_2049x513"]
+        out["inflow_type3"] = converged_comparison(mk_mesh(), mk_flow(), 4000)
+        print(f"bench: ramp_2049x513 (type-3 inflow) {out['inflow_type3']}", file=sys.stderr,
+              flush=True)
+        mk_mesh, mk_flow = gridgen.WORKLOADS["ramp_2049x513_t7"]
+        out["converged"] = {"workload": f"ramp_2049x513_t7: {DESCRIBE['ramp_2049x513_t7']}",
+                            **converged_comparison(mk_mesh(), mk_flow(), converge)}
+        print(f"bench: ramp_2049x513_t7 converged comparison {out['converged']}",
+              file=sys.stderr, flush=True)
+    return out
+
+
+def small_case(stream_gbs, converge=2000, cpu=True):
+    """The 97x49 channel / flat plate (4753 cells, the case the reference runs on
+    one CPU core): its NITER=2000 iterations timed on the GPU, beside the
+    reference."""
+    return other_case("channel_97x49", stream_gbs, 2000, 200, converge, cpu)
 
 
 def batch_sweep(ncases=64, steps=100, warmup=20, seq_steps=50, precision="fp32-mixed",
@@ -411,6 +472,8 @@ def main():
                     help="with --batch: skip the one-after-another runs")
     ap.add_argument("--no-large", action="store_true",
                     help="skip the 2049x513 ramp roofline run beside the main workload")
+    ap.add_argument("--no-small", action="store_true",
+                    help="skip the 97x49 channel run beside the main workload")
     ap.add_argument("--converge", type=int, default=-1, metavar="ITERS",
                     help="iterations of the converged FP32-mixed vs FP64 comparison "
                          "(default: the case's NITER; 0 skips it)")
@@ -499,8 +562,11 @@ def main():
     roof["stream_GBs"] = stream
     if stream:
         roof["frac_of_stream"] = round(roof["achieved"] / stream, 4)
-    large = None if args.no_large or args.workload == "ramp_2049x513" else large_case(stream)
     cpu = None if args.no_cpu else cpu_baseline(mesh, flow)
+    large = None if args.no_large or args.workload.startswith("ramp_2049x513") else \
+        large_case(stream, converge=0 if args.converge == 0 else 20000, cpu=not args.no_cpu)
+    small = None if args.no_small or args.workload == "channel_97x49" else \
+        small_case(stream, converge=0 if args.converge == 0 else 2000, cpu=not args.no_cpu)
     line = {
         "metric": "Mcell-iters/s on 1 MI355X, FP64 vs FP32-mixed at fixed error; % HBM peak",
         "value": round(v64, 2), "unit": "Mcell-iters/s", "n_gpus": 1, "steps": args.steps,
@@ -514,6 +580,7 @@ def main():
         "fp32_mixed": mixed,
         "roofline": roof,
         "large_case": large,
+        "small_case": small,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -16,7 +16,9 @@
 #   * all five physics modules: drop the private, never-assigned
 #     "INTEGER :: IAXIS" and use SOLVER_STATE's IAXIS (the copy flow.in sets)
 #   * turbnstwod.f90 : drop "DATA IAXIS" (flow.in overrides it anyway)
-#   * optional NN override (REF_NN=...) for grids above 60000 cells+ghosts
+#   * NN = 60000 (the shipped value) for turbnstwod / ref_driver, and a second
+#     build with NN = REF_NN_BIG (default 1060000, enough for the 2049x513
+#     ramp's cells + ghosts; 0 skips it) for turbnstwod_big / ref_driver_big
 # Build flags: -O3 only.  The reference Makefile's -fdefault-real-8 promotes
 # DOUBLE PRECISION to REAL(16); the oracle is the FP64 build.
 set -euo pipefail
@@ -70,7 +72,8 @@ build_variant() {   # $1 = NN value, $2 = suffix
 }
 
 build_variant 60000 ""
-if [ -n "${REF_NN_BIG:-}" ]; then
+REF_NN_BIG=${REF_NN_BIG:-1060000}
+if [ "$REF_NN_BIG" != "0" ]; then
     build_variant "$REF_NN_BIG" "_big"
 fi
 echo "build_ref.sh: built $(ls "$OUT" | tr '\n' ' ')"

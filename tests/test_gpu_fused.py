@@ -280,6 +280,28 @@ def test_fp32_mixed_error_at_duct_run_length(monkeypatch):
     assert err["p"] <= 1e-5 and err["rho"] <= 1e-5, err
 
 
+def test_fp32_mixed_error_at_ramp_run_length():
+    """The FP32-mixed path against FP64 on the 2049x513 ramp with the type-7
+    (fixed p, T, u) Mach 2.5 inflow after the case's 20000 iterations: both
+    stay finite and the pressure is within 1e-5 relative L2.  (With the type-3
+    inflow the case goes non-finite before 4000 iterations in either precision;
+    bench.py reports where.)"""
+    import tdt
+    from tdt import analysis, gridgen
+    mk_mesh, mk_flow = gridgen.WORKLOADS["ramp_2049x513_t7"]
+    mesh, flow = mk_mesh(), mk_flow()
+    states = {}
+    for prec in ("fp64", "fp32-mixed"):
+        with tdt.Solver(mesh, flow, prec) as s:
+            s.cold_start()
+            _, _, bad = s.iterate(flow.niter, history=False)
+            assert bad == 0
+            states[prec] = s.get_state()
+    err = analysis.compare(mesh, flow, states["fp32-mixed"], states["fp64"])
+    print("fp32-mixed vs fp64 after", flow.niter, "iterations:", err)
+    assert err["p"] <= 1e-5, err
+
+
 @pytest.fixture(scope="module")
 def full_ramp():
     """The 2049x513 compression ramp (1,051,137 cells), the roofline workload."""

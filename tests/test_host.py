@@ -194,3 +194,41 @@ def test_sweep_cases_are_reproducible_and_in_range():
         assert 65 * 33 <= m.neles <= 129 * 65
         if fam == "channel":
             assert f.deltabl > 0 and f.aminf == f.amachinit
+
+
+def test_ramp_with_type7_inflow():
+    """The type-7-inflow ramp is the 2049x513 ramp with only its left ghosts
+    changed (type 3 -> 7), and its inlet values are the Mach 2.5 free stream:
+    PINF = P0 / (1 + (g-1)/2 M^2)^(g/(g-1))."""
+    from tdt import gridgen
+    m3 = gridgen.WORKLOADS["ramp_2049x513"][0]()
+    m7 = gridgen.WORKLOADS["ramp_2049x513_t7"][0]()
+    f3 = gridgen.WORKLOADS["ramp_2049x513"][1]()
+    f7 = gridgen.WORKLOADS["ramp_2049x513_t7"][1]()
+    assert np.array_equal(m3.x_mm, m7.x_mm) and np.array_equal(m3.y_mm, m7.y_mm)
+    assert np.array_equal(m3.nc, m7.nc) and np.array_equal(m3.parent, m7.parent)
+    left = m3.side == 4
+    assert left.sum() == 513
+    assert np.all(m3.btype[left] == 3) and np.all(m7.btype[left] == 7)
+    assert np.array_equal(m3.btype[~left], m7.btype[~left])
+    assert f7.aminf == 2.5 and f7.deltabl > 0.0
+    assert abs(f7.pinf - 1.7e6 / 2.25 ** 3.5) < 1e-6 * f7.pinf
+    assert (f7.p0, f7.t0, f7.amachinit, f7.niter) == (f3.p0, f3.t0, f3.amachinit, f3.niter)
+
+
+def test_wall_faces_of_each_workload():
+    """No-slip faces for the wall shear comparison: the channel's and the ramp's
+    whole bottom (planar walls at y = 0 included), the duct's upper wall only
+    (its type-5 axis at y = 0 is not a wall)."""
+    from tdt import analysis, gridgen
+    for name, want in (("channel_97x49", 97), ("ramp_2049x513", 2049), ("duct_385x129", 385)):
+        mk_mesh, mk_flow = gridgen.WORKLOADS[name]
+        m, f = mk_mesh(), mk_flow()
+        faces = analysis.wall_faces(m, f.iaxis)
+        assert len(faces) == want, name
+        ymid = 0.5 * (m.y_mm[faces[:, 1]] + m.y_mm[faces[:, 2]])
+        if f.iaxis:
+            assert np.all(ymid > 1.0), name          # the upper wall, radius >= 25 mm
+        else:
+            assert np.all(ymid <= 600.0 * np.tan(np.radians(15.0)) * 1.0001), name  # ramp top
+            assert np.sum(ymid == 0.0) >= 97 * (name == "channel_97x49") + 819 * (name != "channel_97x49")
