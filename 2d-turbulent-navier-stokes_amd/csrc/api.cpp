@@ -24,6 +24,7 @@ thread_local std::string g_create_error;
 void set_rinit(tdt_ctx *c, double rinit) {
     c->rinit = rinit;
     double rtol = rinit * c->params_rtred;           // RTOLERANCE = RINIT*RTRED
+    if (rtol != c->K.rtol) tdt::fused_drop_graphs(c);
     c->K.rtol = rtol;
     c->K.rtol_rtred = rtol * c->params_rtred;        // RTOLERANCE*RTRED (viscous.f90:120)
     c->Kf.rtol = (float)rtol;
@@ -269,6 +270,7 @@ int tdt_destroy(tdt_ctx *c) {
     if (c->stream) hipStreamDestroy(c->stream);
     delete c->batch;
     delete c->adj;
+    delete c->iadj;
     delete c;
     return TDT_OK;
 }

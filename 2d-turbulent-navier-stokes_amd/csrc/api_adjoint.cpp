@@ -11,7 +11,8 @@
 
 using namespace tdt::api;
 
-namespace {
+namespace tdt {
+namespace api {
 
 int adj_setup(tdt_ctx *c) {
     if (c->adj) return TDT_OK;
@@ -115,7 +116,8 @@ int download_cells(tdt_ctx *c, const double *src, double *dst) {
     return TDT_OK;
 }
 
-}  // namespace
+}  // namespace api
+}  // namespace tdt
 
 extern "C" {
 

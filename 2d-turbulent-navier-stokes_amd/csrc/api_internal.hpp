@@ -1,6 +1,7 @@
 // Helpers shared by the translation units of the C ABI (api.cpp: contexts,
 // state, routine entry points, arrays; api_batch.cpp: batches of cases;
-// api_adjoint.cpp: the residual, its adjoint and parameter/node updates).
+// api_adjoint.cpp: the residual, its adjoint and parameter/node updates;
+// api_adjoint_iter.cpp: the adjoint of whole iterations).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -96,6 +97,12 @@ int init_device(tdt_ctx *c, const tdt_params *P);
 // Uniform initial state of a cold start (turbnstwod.f90:53-71) with the host's
 // libm, as the reference: q = C1..C5, *rinit = RINIT.
 void cold_state(const tdt_params &P, const Consts<double> &K, double q[5], double *rinit);
+
+// Residual adjoint buffers (api_adjoint.cpp), allocated on first use.
+int adj_setup(tdt_ctx *c);
+// [5][neles] in grid.in cell order (host) <-> [5][n] internal order (device)
+int upload_cells(tdt_ctx *c, const double *src, double *dst);
+int download_cells(tdt_ctx *c, const double *src, double *dst);
 
 }  // namespace api
 }  // namespace tdt

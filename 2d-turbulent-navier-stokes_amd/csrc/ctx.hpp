@@ -85,6 +85,31 @@ struct AdjBufs {
     int2 *nlist = nullptr;
 };
 
+// Buffers of the iteration adjoint (adjoint_iter.hip, api_adjoint_iter.cpp):
+// per-cell member lists of the dissipation / viscous / timestep operators and
+// their transpose, the per-member Jacobian blocks, the stage states and
+// residuals of the iteration being reversed, the adjoint work vectors, and the
+// checkpoints of the unrolled forward run.
+struct IterAdjBufs {
+    int *mptr = nullptr, *mlist = nullptr;   // [n + 1], [members]
+    int *tptr = nullptr;                     // [n + 1]
+    int2 *tlist = nullptr;                   // (cell i, member entry e) by member cell
+    bool xy_ok = true;                       // node derivatives available on this mesh
+    double *JE = nullptr, *JEp = nullptr, *JEx = nullptr;
+    double *qn = nullptr, *qs[3] = {nullptr, nullptr, nullptr}, *r[4] = {};
+    double *mu = nullptr, *lam = nullptr, *wr = nullptr, *W = nullptr;
+    double *gdt = nullptr, *gvol = nullptr, *grtol = nullptr, *g = nullptr;
+    double *part = nullptr, *gp = nullptr, *gxy = nullptr;   // gp: P0, T0, RTOL
+    // unrolled run: checkpoints every `stride` iterations, one segment of states,
+    // the final state
+    std::vector<double *> ck;
+    std::vector<double *> seg;
+    double *qlast = nullptr;
+    int niter = 0, stride = 0;
+    bool cold = false;
+    double rinit = 0.0;
+};
+
 }  // namespace tdt
 
 struct tdt_ctx {
@@ -107,6 +132,7 @@ struct tdt_ctx {
     double *post = nullptr;   // [6][m] POSTPROCESS output, allocated on first use
     tdt::BatchInfo *batch = nullptr;   // tdt_batch_create contexts only
     tdt::AdjBufs *adj = nullptr;       // residual adjoint buffers, allocated on first use
+    tdt::IterAdjBufs *iadj = nullptr;  // iteration adjoint buffers, allocated on first use
 };
 
 namespace tdt {
@@ -136,9 +162,29 @@ hipError_t launch_res_vjp(tdt_ctx *c, const double *qs, const double *v, const i
                           double *gp, double *Jx, const int *nptr, const int2 *nlist,
                           double *gxy);
 
+// Iteration adjoint (adjoint_iter.hip); states [5][n] internal order.
+// od: [D1..D5][n], ov: [VIS1..VIS4, DT][n]
+hipError_t launch_op_eval(tdt_ctx *c, const double *qs, double *od, double *ov);
+hipError_t launch_op_jac(tdt_ctx *c, const double *qs, const int *mptr, const int *mlist,
+                         double *JE, double *JEp, double *JEx);
+hipError_t launch_op_vjp(tdt_ctx *c, const double *JE, const double *JEp, const double *JEx,
+                         const double *W, const double *gdt, const double *gvol,
+                         const double *grtol, const int *tptr, const int2 *tlist,
+                         const int *nptr, const int2 *nlist, double *part, double *g,
+                         double *gp, double *gxy);
+hipError_t launch_stage_vjp(tdt_ctx *c, double alpha, const double *q, const double *r,
+                            const double *vis, const double *d24, const double *dt,
+                            const double *vol, const double *mu, double *lam, double *wr,
+                            double *W, double *gdt, double *gvol, double *grtol);
+hipError_t launch_add(tdt_ctx *c, int count, const double *x, double *y);
+// the cells' C rows [5][nt] -> [5][n]; and [5][n] -> C and COLD
+hipError_t launch_pack(tdt_ctx *c, const double *src, double *dst);
+hipError_t launch_unpack(tdt_ctx *c, const double *src);
+
 // Fused path (fused.hip).
 int fused_setup(tdt_ctx *c);
 void fused_free(tdt_ctx *c);
+void fused_drop_graphs(tdt_ctx *c);   // constants changed: recapture on the next iterate
 int fused_iterate(tdt_ctx *c, int niter, double *l2, double *linf, int *first_nonfinite);
 int fused_kernel_set(tdt_ctx *c);   // kernel set in use, -1 before the first iterate
 

@@ -117,12 +117,16 @@ __device__ __forceinline__ Dual<N> &operator-=(Dual<N> &a, const Dual<N> &b) { r
 template <int N>
 __device__ __forceinline__ Dual<N> fabs(const Dual<N> &a) { return a.v < 0.0 ? -a : a; }
 
+// sqrt' = 1 / (2 sqrt); at exactly 0 the derivatives are set to 0: there the
+// argument is a square-like quantity (the turbulence production of a uniform
+// state) whose own derivative is 0, so sqrt of it behaves like |x|, which has no
+// derivative at 0 and whose central differences are 0
 template <int N>
 __device__ __forceinline__ Dual<N> tsqrt(const Dual<N> &a) {
     Dual<N> r;
     r.v = sqrt(a.v);
 #pragma unroll
-    for (int k = 0; k < N; ++k) r.d[k] = a.d[k] / (r.v + r.v);
+    for (int k = 0; k < N; ++k) r.d[k] = r.v > 0.0 ? a.d[k] / (r.v + r.v) : 0.0;
     return r;
 }
 

@@ -261,6 +261,14 @@ static void drop_graphs(FusedBufs &B) {
     if (B.graphK) { (void)hipGraphDestroy(B.graphK); B.graphK = nullptr; }
 }
 
+// The graphs hold the flow constants as kernel arguments: after a change of
+// RINIT (RTOL) they are captured again on the next iterate.
+void fused_drop_graphs(tdt_ctx *c) {
+    if (!c->fused) return;
+    (void)hipStreamSynchronize(c->stream);
+    drop_graphs(*c->fused);
+}
+
 // Runs `iters` iterations of kernel set `kset` eagerly.  With per_slot, each
 // kernel is bracketed by events and per_slot[q] receives the average time of
 // slot q; otherwise *total_ms receives the whole run's time.

@@ -1,4 +1,5 @@
-"""torch.autograd binding of the residual and its adjoint.
+"""torch.autograd bindings of the residual and of whole iterations, with their
+adjoints as the backward passes.
 
 `residual(q, p, solver, xy)` returns R(q) for the state q (float64 tensor
 [5, neles]: C1..C5 in grid.in cell order) with p = (P0, T0, AMACHINIT) as the
@@ -9,6 +10,12 @@ variables, boundary conditions and source terms of that state
 (tdt_residual_eval).  Its backward is the library's adjoint
 (tdt_residual_vjp): v^T dR/dQ, v^T dR/dp and v^T dR/d(X, Y), computed on the GPU
 by transposed stencil gathers.
+
+`unrolled(p, solver, niter, xy)` returns Q^N ([5, neles]) after niter
+iterations from the cold start of p = (P0, T0, AMACHINIT) on the node
+coordinates xy (tdt_adjoint_forward); its backward is the reverse sweep
+(tdt_adjoint_reverse), which recomputes each iteration's RK stages from states
+kept every ~sqrt(niter) iterations.
 
 The tensors stay on the CPU: the library holds the device buffers, so torch is
 the autograd plumbing here, not a compute path.  libtdt is loaded before torch
@@ -69,3 +76,44 @@ def residual(q, p, solver, xy=None):
     if xy is None:
         xy = torch.from_numpy(solver.xy_mm.copy())
     return Residual.apply(q, p, xy, solver)
+
+
+_runs = [0]   # labels of forward runs, so a backward can tell whether its run is the last
+
+
+class Unrolled(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, p, xy, solver, niter):
+        _use_params(solver, p)
+        _use_nodes(solver, xy)
+        qn = solver.adjoint_forward(int(niter))
+        _runs[0] += 1
+        ctx.run = _runs[0]
+        solver._adjoint_run = ctx.run
+        ctx.solver = solver
+        ctx.niter = int(niter)
+        ctx.save_for_backward(p, xy)
+        return torch.from_numpy(qn).to(dtype=p.dtype)
+
+    @staticmethod
+    def backward(ctx, grad_qn):
+        p, xy = ctx.saved_tensors
+        solver = ctx.solver
+        _use_params(solver, p)
+        _use_nodes(solver, xy)
+        if getattr(solver, "_adjoint_run", None) != ctx.run:   # another run came since
+            solver.adjoint_forward(ctx.niter)
+            solver._adjoint_run = ctx.run
+        _, gp, gxy = solver.adjoint_reverse(grad_qn.detach().cpu().numpy().astype(np.float64),
+                                            nodes=True)
+        return (torch.from_numpy(gp[:3].copy()).to(dtype=p.dtype),
+                torch.from_numpy(gxy).to(dtype=xy.dtype), None, None)
+
+
+def unrolled(p, solver, niter, xy=None):
+    """Q^N after niter iterations from the cold start of p = (P0, T0, AMACHINIT)
+    on the node coordinates xy (default: the solver's), with the reverse sweep
+    as its backward."""
+    if xy is None:
+        xy = torch.from_numpy(solver.xy_mm.copy())
+    return Unrolled.apply(p, xy, solver, niter)

@@ -100,6 +100,11 @@ def load(path: str = LIB_PATH):
     L.tdt_batch_set_state.argtypes = [vp, ctypes.c_int32, _dp, ctypes.c_double]
     L.tdt_batch_get_state.argtypes = [vp, ctypes.c_int32, _dp]
     L.tdt_batch_iterate.argtypes = [vp, ctypes.c_int32, _dp, _dp, _ip]
+    L.tdt_operators_eval.argtypes = [vp, _dp, _dp]
+    L.tdt_operators_vjp.argtypes = [vp, _dp, ctypes.c_double, _dp, _dp, _dp, _dp]
+    L.tdt_iteration_vjp.argtypes = [vp, _dp, ctypes.c_double, _dp, _dp, _dp, _dp]
+    L.tdt_adjoint_forward.argtypes = [vp, ctypes.c_int32, _dp, ctypes.c_double, _dp]
+    L.tdt_adjoint_reverse.argtypes = [vp, _dp, _dp, _dp, _dp]
     _lib = L
     return L
 
@@ -254,6 +259,66 @@ class Solver:
         self._check(self.L.tdt_residual_vjp(self.h, _d(c), _d(v), _d(g), _d(gp),
                                             _d(gxy) if nodes else None))
         return (g, gp, gxy) if nodes else (g, gp)
+
+    # ---- adjoint of whole iterations ----------------------------------------------------
+    def operators_of(self, c: np.ndarray) -> dict:
+        """D241..D245, VISU..VISR and DT of the state c as the iteration adjoint
+        evaluates them (the routine entry points' values)."""
+        c = np.ascontiguousarray(c, dtype=np.float64).reshape(5, self.neles)
+        out = np.empty((10, self.neles))
+        self._check(self.L.tdt_operators_eval(self.h, _d(c), _d(out)))
+        names = [f"D24{m}" for m in range(1, 6)] + ["VISU", "VISV", "VIST", "VISR", "DT"]
+        return dict(zip(names, out))
+
+    def operators_vjp(self, c: np.ndarray, rinit: float, w: np.ndarray, nodes: bool = False):
+        """Weights w [7, neles] on E = D + VIS (5 rows), DT and VOL at the state c:
+        (w^T d(E, DT)/dQ [5, neles], w^T dE/d(P0, T0, RINIT) [3]) and with
+        nodes=True also w^T d(E, DT, VOL)/d(X, Y) [2, nods] per millimetre."""
+        c = np.ascontiguousarray(c, dtype=np.float64).reshape(5, self.neles)
+        w = np.ascontiguousarray(w, dtype=np.float64).reshape(7, self.neles)
+        g = np.empty((5, self.neles))
+        gp = np.zeros(3)
+        gxy = np.empty((2, self.nods)) if nodes else None
+        self._check(self.L.tdt_operators_vjp(self.h, _d(c), float(rinit), _d(w), _d(g), _d(gp),
+                                             _d(gxy) if nodes else None))
+        return (g, gp, gxy) if nodes else (g, gp)
+
+    def iteration_vjp(self, c: np.ndarray, rinit: float, mu: np.ndarray, nodes: bool = False):
+        """One iteration Q^n = c -> Q^{n+1}: (mu^T dQ^{n+1}/dQ^n [5, neles],
+        mu^T dQ^{n+1}/d(P0, T0, RINIT) [3]) and with nodes=True also
+        mu^T dQ^{n+1}/d(X, Y) [2, nods] per grid.in millimetre."""
+        c = np.ascontiguousarray(c, dtype=np.float64).reshape(5, self.neles)
+        mu = np.ascontiguousarray(mu, dtype=np.float64).reshape(5, self.neles)
+        lam = np.empty((5, self.neles))
+        gp = np.zeros(3)
+        gxy = np.empty((2, self.nods)) if nodes else None
+        self._check(self.L.tdt_iteration_vjp(self.h, _d(c), float(rinit), _d(mu), _d(lam),
+                                             _d(gp), _d(gxy) if nodes else None))
+        return (lam, gp, gxy) if nodes else (lam, gp)
+
+    def adjoint_forward(self, niter: int, c0=None, rinit: float = 0.0) -> np.ndarray:
+        """niter iterations from c0 (None: the cold start), keeping states for
+        adjoint_reverse; returns Q^N [5, neles]."""
+        cn = np.empty((5, self.neles))
+        if c0 is None:
+            self._check(self.L.tdt_adjoint_forward(self.h, int(niter), None, 0.0, _d(cn)))
+        else:
+            c0 = np.ascontiguousarray(c0, dtype=np.float64).reshape(5, self.neles)
+            self._check(self.L.tdt_adjoint_forward(self.h, int(niter), _d(c0), float(rinit),
+                                                   _d(cn)))
+        return cn
+
+    def adjoint_reverse(self, dj_dcn: np.ndarray, nodes: bool = False):
+        """Reverse sweep of the last adjoint_forward: (dJ/dQ^0 [5, neles],
+        dJ/d(P0, T0, AMACHINIT, RINIT) [4]) and with nodes=True also dJ/d(X, Y)
+        [2, nods] per grid.in millimetre."""
+        g = np.ascontiguousarray(dj_dcn, dtype=np.float64).reshape(5, self.neles)
+        lam = np.empty((5, self.neles))
+        gp = np.zeros(4)
+        gxy = np.empty((2, self.nods)) if nodes else None
+        self._check(self.L.tdt_adjoint_reverse(self.h, _d(g), _d(lam), _d(gp),
+                                               _d(gxy) if nodes else None))
+        return (lam, gp, gxy) if nodes else (lam, gp)
 
     def set_nodes(self, x_mm: np.ndarray, y_mm: np.ndarray) -> None:
         """New node coordinates (grid.in order, mm); the geometry is recomputed."""

@@ -148,6 +148,47 @@ int tdt_set_params(tdt_ctx *ctx, const tdt_params *params);
  * geometry (geometry.f90:55-149) is recomputed. */
 int tdt_set_nodes(tdt_ctx *ctx, const double *x_mm, const double *y_mm);
 
+/* ---- adjoint of whole iterations ---------------------------------------------------
+ * No counterpart in the reference: the adjoint of the RK4 iteration of
+ * turbnstwod.f90:95-145 (time step, DISSIPATION, the four stages with
+ * COMPUTE_VISCOUS at stage 1, UPDATE_SOLUTION and its nu~ floor), FP64, with the
+ * routine entry points' arithmetic.  Needs LOCAL = 1; BC type 6 grids and batch
+ * contexts: TDT_ERR_UNSUPPORTED.  States, adjoints: [5][neles], grid.in order.
+ *
+ * tdt_operators_eval: out = [10][neles] = D241..D245, VISU, VISV, VIST, VISR, DT
+ * of the state c as the adjoint evaluates them (the same bits as the routine
+ * entry points after tdt_timestep, tdt_dissipation, tdt_primitives,
+ * tdt_apply_boundary and tdt_viscous on c). */
+int tdt_operators_eval(tdt_ctx *ctx, const double *c, double *out);
+/* Adjoint of those operators at the state c (with RINIT = rinit): for weights
+ * w = [7][neles] on E = (D241, D242 + VISU, D243 + VISV, D244 + VIST,
+ * D245 + VISR), DT and VOL, g = w^T d(E, DT)/dQ ([5][neles]); gparam ([3] or
+ * NULL) = w^T dE/d(P0, T0, RINIT); gxy ([2][nods] or NULL) = w^T d(E, DT,
+ * VOL)/d(X, Y) per grid.in millimetre. */
+int tdt_operators_vjp(tdt_ctx *ctx, const double *c, double rinit, const double *w, double *g,
+                      double *gparam, double *gxy);
+/* One iteration Q^n = c (with RINIT = rinit) -> Q^{n+1}: lam = mu^T dQ^{n+1}/dQ^n;
+ * gparam ([3] or NULL) = mu^T dQ^{n+1}/d(P0, T0, RINIT); gxy ([2][nods] or NULL)
+ * = mu^T dQ^{n+1}/d(X, Y) per grid.in millimetre.  The context's C is kept (COLD
+ * is set to it); its routine-level arrays are overwritten. */
+int tdt_iteration_vjp(tdt_ctx *ctx, const double *c, double rinit, const double *mu,
+                      double *lam, double *gparam, double *gxy);
+/* niter iterations from c0 with RINIT = rinit (c0 NULL: from the cold start of
+ * turbnstwod.f90:53-71, rinit ignored), as tdt_iterate computes them; cn = Q^N.
+ * States are kept every ~sqrt(niter) iterations for tdt_adjoint_reverse.  The
+ * context's state ends at Q^N. */
+int tdt_adjoint_forward(tdt_ctx *ctx, int32_t niter, const double *c0, double rinit,
+                        double *cn);
+/* Reverse sweep of the last tdt_adjoint_forward, given dj_dcn = dJ/dQ^N:
+ * lam0 = dJ/dQ^0; gparam ([4] or NULL) = dJ/d(P0, T0, AMACHINIT, RINIT) -- after
+ * a cold start the first three include the path through Q^0 and RINIT (which
+ * the cold start computes from them) and the fourth is the part through RINIT
+ * alone; from a given c0 they hold Q^0 and RINIT fixed (AMACHINIT: 0) --; gxy
+ * ([2][nods] or NULL) = dJ/d(X, Y) per grid.in millimetre.  The stage states of
+ * each iteration are recomputed from the kept states, not stored. */
+int tdt_adjoint_reverse(tdt_ctx *ctx, const double *dj_dcn, double *lam0, double *gparam,
+                        double *gxy);
+
 /* ---- batched cases ----------------------------------------------------------------
  * No counterpart in the reference, which runs one case per process: for parameter
  * sweeps, ncases independent grids advanced together.  One fused iteration (the

@@ -362,3 +362,24 @@ def test_full_size_ramp_orderings_and_precisions(full_ramp):
     for k in ("rho", "p"):
         err = np.linalg.norm(a[k] - b[k]) / np.linalg.norm(b[k])
         assert err < 1e-5, (k, err)
+
+
+def test_new_rinit_after_iterate_is_used(golden):
+    """tdt_set_state with another RINIT after tdt_iterate: the captured graphs
+    hold the flow constants (RTOL = RINIT * RTRED), so they must be captured
+    again; the run then equals a fresh context's from the same state."""
+    import tdt
+    g = golden("t2d_chan")
+    with tdt.Solver(g.mesh, g.flow) as a, tdt.Solver(g.mesh, g.flow) as b:
+        rinit = a.cold_start()
+        a.iterate(3, history=False)
+        c = g["final"]
+        for scale in (1e3, 1.0):
+            a.set_state(c, rinit * scale)
+            a.iterate(3, history=False)
+            b.set_state(c, rinit * scale)
+            b.iterate(3, history=False)
+            assert np.array_equal(a.get_state(), b.get_state()), scale
+        a.set_state(c, rinit * 1e3)
+        a.iterate(3, history=False)
+        assert not np.array_equal(a.get_state(), b.get_state())
