@@ -7,8 +7,8 @@
 // is built from its parent, which is i, and for type 3 from the neighbour
 // across i's opposite face, itself one of i's neighbours).  So dR/dQ is a
 // block-sparse matrix with one 5x5 block per (cell, stencil member):
-//   k_res_jac  forms, per cell, the five blocks dR_i/dQ_s (s = i, then the
-//              neighbours across faces 1..4) and dR_i/d(P0, T0), by forward-
+//   k_res_jac  forms, per cell (one thread per block), the five blocks dR_i/dQ_s
+//              (s = i, then the neighbours across faces 1..4) and dR_i/d(P0, T0), by forward-
 //              mode dual numbers through the same physics functions the solver
 //              runs (phys.hpp), so the value part is the solver's R bit for bit;
 //   k_res_vjp  applies the transpose as a gather: g_k = sum over the (i, s)
@@ -201,12 +201,15 @@ struct ParamSeeds {
     double ds0_dp0, ds0_dt0, dtinf_dt0, dblu_dt0;
 };
 
-// J[i] = [s][m][c]: dR_i,m / dQ_s,c (zero blocks for ghost faces); Jp[i] =
-// [m][p]: dR_i,m / d(P0, T0).
+// One dual evaluation per thread, blockIdx.y = e: member e < 5 (J[i] block e =
+// dR_i,m / dQ_s,c [m][c], zero for a ghost face), the parameters (e == 5: Jp[i] =
+// [m][p]: dR_i,m / d(P0, T0)) or node pass e - 6 (Jx[i], five of the 40
+// directions (slot, x or y)).
 __global__ void __launch_bounds__(kBlock) k_res_jac(Dev D, Consts<double> K, ParamSeeds ps,
                                                     const double *qs, double *J, double *Jp,
                                                     double *Jx) {
     const int i = gid();
+    const int e = blockIdx.y;
     if (i >= D.n) return;
     const int n = D.n;
     const int4 nb4 = D.nc4[i];
@@ -214,65 +217,59 @@ __global__ void __launch_bounds__(kBlock) k_res_jac(Dev D, Consts<double> K, Par
     double qd[5][5];
     load_stencil(D, qs, i, qd);
     const Consts<Dual<kDualN>> KD = lift<kDualN>(K);
-    CellGeo<Dual<kDualN>> GD;
-    geo_stored(D, i, GD);
-    double *Ji = J + (size_t)i * kJacStride;
-    for (int s = 0; s < kJacBlocks; ++s) {
-        if (s > 0 && nb[s - 1] >= n) {
-            for (int e = 0; e < 25; ++e) Ji[s * 25 + e] = 0.0;
-            continue;
-        }
-        Dual<kDualN> q[5][5], r[5];
-        for (int a = 0; a < 5; ++a)
-            for (int c = 0; c < 5; ++c) q[a][c] = Dual<kDualN>(qd[a][c]);
-        for (int c = 0; c < 5; ++c) q[s][c].d[c] = 1.0;   // direction c: Q_s,c
-        cell_residual<Dual<kDualN>>(D, i, q, GD, KD, r);
-        for (int m = 0; m < 5; ++m)
-            for (int c = 0; c < 5; ++c) Ji[s * 25 + m * 5 + c] = r[m].d[c];
-    }
-    // parameters: directions 0 (P0) and 1 (T0) of the constants that hold them
-    Consts<Dual<kDualN>> KP = KD;
-    KP.s0.d[0] = ps.ds0_dp0;
-    KP.s0.d[1] = ps.ds0_dt0;
-    KP.t0.d[1] = 1.0;
-    KP.tinf.d[1] = ps.dtinf_dt0;
-    KP.bl_u.d[1] = ps.dblu_dt0;
     Dual<kDualN> q[5][5], r[5];
     for (int a = 0; a < 5; ++a)
         for (int c = 0; c < 5; ++c) q[a][c] = Dual<kDualN>(qd[a][c]);
-    cell_residual<Dual<kDualN>>(D, i, q, GD, KP, r);
-    for (int m = 0; m < 5; ++m) {
-        Jp[(size_t)i * 10 + m * 2] = r[m].d[0];
-        Jp[(size_t)i * 10 + m * 2 + 1] = r[m].d[1];
+    if (e < kJacBlocks) {
+        double *Ji = J + (size_t)i * kJacStride + e * 25;
+        if (e > 0 && nb[e - 1] >= n) {
+            for (int k = 0; k < 25; ++k) Ji[k] = 0.0;
+            return;
+        }
+        CellGeo<Dual<kDualN>> GD;
+        geo_stored(D, i, GD);
+        for (int c = 0; c < 5; ++c) q[e][c].d[c] = 1.0;   // direction c: Q_e,c
+        cell_residual<Dual<kDualN>>(D, i, q, GD, KD, r);
+        for (int m = 0; m < 5; ++m)
+            for (int c = 0; c < 5; ++c) Ji[m * 5 + c] = r[m].d[c];
+        return;
+    }
+    if (e == kJacBlocks) {   // parameters: directions 0 (P0) and 1 (T0)
+        Consts<Dual<kDualN>> KP = KD;
+        KP.s0.d[0] = ps.ds0_dp0;
+        KP.s0.d[1] = ps.ds0_dt0;
+        KP.t0.d[1] = 1.0;
+        KP.tinf.d[1] = ps.dtinf_dt0;
+        KP.bl_u.d[1] = ps.dblu_dt0;
+        CellGeo<Dual<kDualN>> GD;
+        geo_stored(D, i, GD);
+        cell_residual<Dual<kDualN>>(D, i, q, GD, KP, r);
+        for (int m = 0; m < 5; ++m) {
+            Jp[(size_t)i * 10 + m * 2] = r[m].d[0];
+            Jp[(size_t)i * 10 + m * 2 + 1] = r[m].d[1];
+        }
+        return;
     }
     if (!Jx) return;
-    // node coordinates: 40 directions (slot, x or y), kDualN at a time
-    double xs[kNodeSlots], ys[kNodeSlots];
+    const int d0 = (e - kJacBlocks - 1) * kDualN;   // node coordinates: this pass's directions
+    if (d0 >= 2 * kNodeSlots) return;
+    Dual<kDualN> x[kNodeSlots], y[kNodeSlots];
     for (int sl = 0; sl < kNodeSlots; ++sl) {
         const int v = slot_node(D, i, sl);
-        xs[sl] = v >= 0 ? D.xn[v] : 0.0;
-        ys[sl] = v >= 0 ? D.yn[v] : 0.0;
+        x[sl] = Dual<kDualN>(v >= 0 ? D.xn[v] : 0.0);
+        y[sl] = Dual<kDualN>(v >= 0 ? D.yn[v] : 0.0);
     }
+    for (int k = 0; k < kDualN; ++k) {   // direction k: slot (d0+k)/2, x if even
+        const int dir = d0 + k, sl = dir >> 1;
+        if (dir & 1) y[sl].d[k] = 1.0;
+        else x[sl].d[k] = 1.0;
+    }
+    CellGeo<Dual<kDualN>> GX;
+    geo_from_nodes(D, i, x, y, K.iaxis, GX);
+    cell_residual<Dual<kDualN>>(D, i, q, GX, KD, r);
     double *Jxi = Jx + (size_t)i * kXyStride;
-    for (int d0 = 0; d0 < 2 * kNodeSlots; d0 += kDualN) {
-        Dual<kDualN> x[kNodeSlots], y[kNodeSlots];
-        for (int sl = 0; sl < kNodeSlots; ++sl) {
-            x[sl] = Dual<kDualN>(xs[sl]);
-            y[sl] = Dual<kDualN>(ys[sl]);
-        }
-        for (int e = 0; e < kDualN; ++e) {   // direction e: slot (d0+e)/2, x if even
-            const int dir = d0 + e, sl = dir >> 1;
-            if (dir & 1) y[sl].d[e] = 1.0;
-            else x[sl].d[e] = 1.0;
-        }
-        CellGeo<Dual<kDualN>> GX;
-        geo_from_nodes(D, i, x, y, K.iaxis, GX);
-        cell_residual<Dual<kDualN>>(D, i, q, GX, KD, r);
-        for (int e = 0; e < kDualN; ++e) {
-            const int dir = d0 + e;
-            for (int m = 0; m < 5; ++m) Jxi[dir * 5 + m] = r[m].d[e];
-        }
-    }
+    for (int k = 0; k < kDualN; ++k)
+        for (int m = 0; m < 5; ++m) Jxi[(d0 + k) * 5 + m] = r[m].d[k];
 }
 
 // gxy_v = sum over the (cell i, slot s) whose slot holds node v, in order, of
@@ -378,8 +375,9 @@ hipError_t launch_res_vjp(tdt_ctx *c, const double *qs, const double *v, const i
     ps.dtinf_dt0 = (double)K.tinf / P.t0;
     ps.dblu_dt0 = 0.5 * (double)K.bl_u / P.t0;
     const unsigned nb = nblocks(c->D.n);
-    hipLaunchKernelGGL(k_res_jac, dim3(nb), dim3(kBlock), 0, c->stream, c->D, K, ps, qs, J, Jp,
-                       gxy ? Jx : nullptr);
+    const unsigned evals = kJacBlocks + 1 + (gxy ? 2 * kNodeSlots / kDualN : 0);
+    hipLaunchKernelGGL(k_res_jac, dim3(nb, evals), dim3(kBlock), 0, c->stream, c->D, K, ps, qs,
+                       J, Jp, gxy ? Jx : nullptr);
     hipLaunchKernelGGL(k_res_vjp, dim3(nb), dim3(kBlock), 0, c->stream, c->D, J, v, tptr, tlist,
                        g);
     hipLaunchKernelGGL(k_par_vjp, dim3(nb), dim3(kBlock), 0, c->stream, c->D, Jp, v, part);

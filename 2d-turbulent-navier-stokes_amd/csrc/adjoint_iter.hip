@@ -13,7 +13,8 @@
 // of cells around i: the face neighbours, their face neighbours (the sensors
 // and D2Q of the dissipation), and the cells and ghost parents of the node
 // averages at i's four nodes.  The host lists these members per cell.
-//   k_op_jac   forms, per cell and member, the 6x5 block d(E_i, DT_i)/dQ_member
+//   k_op_jac   forms, per cell and member (one thread each), the 6x5 block
+//              d(E_i, DT_i)/dQ_member
 //              by forward-mode dual numbers through the same arithmetic as the
 //              routine kernels (k_timestep, k_dissip1/2, k_prim, k_bc,
 //              k_nodeavg, k_viscous), so the value part is theirs bit for bit;
@@ -405,37 +406,44 @@ struct OpSeeds {
     double ds0_dp0, ds0_dt0, dtinf_dt0, dblu_dt0, rtred;
 };
 
-// JE[mptr[i] + s] = d(E_i, DT_i)/dQ_(member s) [row][c]; JEp[i] = dE_i/d(P0, T0,
-// RTOL) [row][p]; JEx[i] (when given) = d(E_i, DT_i, VOL_i)/d(slot x, y).
+// One dual evaluation per thread: blockIdx.y = e selects, for cell i, member e
+// (e < its member count: JE[mptr[i] + e] = d(E_i, DT_i)/dQ_(member) [row][c]),
+// the parameters (e == kmax: JEp[i] = dE_i/d(P0, T0, RTOL) [row][p]) or node
+// pass e - kmax - 1 (JEx[i], d(E_i, DT_i, VOL_i)/d(slot x, y) for the five
+// directions of that pass).  kmax = the largest member count.
 __global__ void __launch_bounds__(64) k_op_jac(Dev D, Consts<double> K, OpSeeds ps,
                                                const double *qs, const int *mptr,
-                                               const int *mlist, double *JE, double *JEp,
-                                               double *JEx) {
+                                               const int *mlist, int kmax, double *JE,
+                                               double *JEp, double *JEx) {
     const int i = blockIdx.x * 64 + threadIdx.x;
+    const int e = blockIdx.y;
     if (i >= D.n) return;
     const int n = D.n;
     const Consts<Dual<kDualN>> KD = lift<kDualN>(K);
-    OpGeoStored<Dual<kDualN>> G{&D, i};
-    for (int e = mptr[i]; e < mptr[i + 1]; ++e) {
+    if (e < kmax) {
+        const int ent = mptr[i] + e;
+        if (ent >= mptr[i + 1]) return;
+        OpGeoStored<Dual<kDualN>> G{&D, i};
         OpOut<Dual<kDualN>> o;
-        cell_ops<Dual<kDualN>>(D, i, SeedLoader{qs, n, mlist[e]}, G, KD, o);
-        double *B = JE + (size_t)e * kOpBlock;
+        cell_ops<Dual<kDualN>>(D, i, SeedLoader{qs, n, mlist[ent]}, G, KD, o);
+        double *B = JE + (size_t)ent * kOpBlock;
         for (int c = 0; c < 5; ++c) {
             B[c] = o.d[0].d[c];
             for (int m = 1; m < 5; ++m) B[m * 5 + c] = o.d[m].d[c] + o.vis[m - 1].d[c];
             B[25 + c] = o.dt.d[c];
         }
+        return;
     }
-    // parameters: directions 0 (P0), 1 (T0), 2 (RTOL)
-    Consts<Dual<kDualN>> KP = KD;
-    KP.s0.d[0] = ps.ds0_dp0;
-    KP.s0.d[1] = ps.ds0_dt0;
-    KP.t0.d[1] = 1.0;
-    KP.tinf.d[1] = ps.dtinf_dt0;
-    KP.bl_u.d[1] = ps.dblu_dt0;
-    KP.rtol.d[2] = 1.0;
-    KP.rtol_rtred.d[2] = ps.rtred;
-    {
+    if (e == kmax) {   // parameters: directions 0 (P0), 1 (T0), 2 (RTOL)
+        Consts<Dual<kDualN>> KP = KD;
+        KP.s0.d[0] = ps.ds0_dp0;
+        KP.s0.d[1] = ps.ds0_dt0;
+        KP.t0.d[1] = 1.0;
+        KP.tinf.d[1] = ps.dtinf_dt0;
+        KP.bl_u.d[1] = ps.dblu_dt0;
+        KP.rtol.d[2] = 1.0;
+        KP.rtol_rtred.d[2] = ps.rtred;
+        OpGeoStored<Dual<kDualN>> G{&D, i};
         OpOut<Dual<kDualN>> o;
         cell_ops<Dual<kDualN>>(D, i, ConstLoader{qs, n}, G, KP, o);
         double *P = JEp + (size_t)i * kOpParRows * 3;
@@ -443,14 +451,16 @@ __global__ void __launch_bounds__(64) k_op_jac(Dev D, Consts<double> K, OpSeeds 
             P[p] = o.d[0].d[p];
             for (int m = 1; m < 5; ++m) P[m * 3 + p] = o.d[m].d[p] + o.vis[m - 1].d[p];
         }
+        return;
     }
     if (!JEx) return;
+    const int d0 = (e - kmax - 1) * kDualN;   // first of this pass's directions
+    if (d0 >= 2 * kNodeSlots) return;
     OpGeoSlots<Dual<kDualN>> GS;
     GS.D = &D;
     GS.i = i;
     const int4 nb4 = D.nc4[i];
     GS.nb[0] = nb4.x; GS.nb[1] = nb4.y; GS.nb[2] = nb4.z; GS.nb[3] = nb4.w;
-    double xs[kNodeSlots], ys[kNodeSlots];
     const int *nod = reinterpret_cast<const int *>(D.nod4);
     for (int sl = 0; sl < kNodeSlots; ++sl) {
         int v;
@@ -461,30 +471,24 @@ __global__ void __launch_bounds__(64) k_op_jac(Dev D, Consts<double> K, OpSeeds 
             v = j < n ? nod[4 * j + ((sl - 4) & 3)] : -1;
         }
         GS.node[sl] = v;
-        xs[sl] = v >= 0 ? D.xn[v] : 0.0;
-        ys[sl] = v >= 0 ? D.yn[v] : 0.0;
+        GS.xs[sl] = Dual<kDualN>(v >= 0 ? D.xn[v] : 0.0);
+        GS.ys[sl] = Dual<kDualN>(v >= 0 ? D.yn[v] : 0.0);
     }
+    for (int k = 0; k < kDualN; ++k) {   // direction k: slot (d0+k)/2, x if even
+        const int dir = d0 + k, sl = dir >> 1;
+        if (dir & 1) GS.ys[sl].d[k] = 1.0;
+        else GS.xs[sl].d[k] = 1.0;
+    }
+    slots_finish(GS, K.iaxis);
+    OpOut<Dual<kDualN>> o;
+    cell_ops<Dual<kDualN>>(D, i, ConstLoader{qs, n}, GS, KD, o);
     double *X = JEx + (size_t)i * kOpXyStride;
-    for (int d0 = 0; d0 < 2 * kNodeSlots; d0 += kDualN) {
-        for (int sl = 0; sl < kNodeSlots; ++sl) {
-            GS.xs[sl] = Dual<kDualN>(xs[sl]);
-            GS.ys[sl] = Dual<kDualN>(ys[sl]);
-        }
-        for (int e = 0; e < kDualN; ++e) {   // direction e: slot (d0+e)/2, x if even
-            const int dir = d0 + e, sl = dir >> 1;
-            if (dir & 1) GS.ys[sl].d[e] = 1.0;
-            else GS.xs[sl].d[e] = 1.0;
-        }
-        slots_finish(GS, K.iaxis);
-        OpOut<Dual<kDualN>> o;
-        cell_ops<Dual<kDualN>>(D, i, ConstLoader{qs, n}, GS, KD, o);
-        for (int e = 0; e < kDualN; ++e) {
-            double *Xr = X + (size_t)(d0 + e) * kOpXyRows;
-            Xr[0] = o.d[0].d[e];
-            for (int m = 1; m < 5; ++m) Xr[m] = o.d[m].d[e] + o.vis[m - 1].d[e];
-            Xr[5] = o.dt.d[e];
-            Xr[6] = o.vol.d[e];
-        }
+    for (int k = 0; k < kDualN; ++k) {
+        double *Xr = X + (size_t)(d0 + k) * kOpXyRows;
+        Xr[0] = o.d[0].d[k];
+        for (int m = 1; m < 5; ++m) Xr[m] = o.d[m].d[k] + o.vis[m - 1].d[k];
+        Xr[5] = o.dt.d[k];
+        Xr[6] = o.vol.d[k];
     }
 }
 
@@ -655,7 +659,7 @@ hipError_t launch_op_eval(tdt_ctx *c, const double *qs, double *od, double *ov) 
 }
 
 hipError_t launch_op_jac(tdt_ctx *c, const double *qs, const int *mptr, const int *mlist,
-                         double *JE, double *JEp, double *JEx) {
+                         int kmax, double *JE, double *JEp, double *JEx) {
     const Consts<double> &K = c->K;
     const tdt_params &P = c->params;
     OpSeeds ps;
@@ -665,8 +669,9 @@ hipError_t launch_op_jac(tdt_ctx *c, const double *qs, const int *mptr, const in
     ps.dtinf_dt0 = (double)K.tinf / P.t0;
     ps.dblu_dt0 = 0.5 * (double)K.bl_u / P.t0;
     ps.rtred = c->params_rtred;
-    hipLaunchKernelGGL(k_op_jac, dim3((c->D.n + 63) / 64), dim3(64), 0, c->stream, c->D, K, ps,
-                       qs, mptr, mlist, JE, JEp, JEx);
+    const unsigned evals = kmax + 1 + (JEx ? 2 * kNodeSlots / kDualN : 0);
+    hipLaunchKernelGGL(k_op_jac, dim3((c->D.n + 63) / 64, evals), dim3(64), 0, c->stream, c->D,
+                       K, ps, qs, mptr, mlist, kmax, JE, JEp, JEx);
     return hipGetLastError();
 }
 

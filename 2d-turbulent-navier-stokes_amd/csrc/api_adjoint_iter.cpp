@@ -87,6 +87,7 @@ int iadj_setup(tdt_ctx *c) {
         mlist.push_back(i);
         mlist.insert(mlist.end(), set.begin(), set.end());
         mptr[i + 1] = (int)mlist.size();
+        b->kmax = std::max(b->kmax, mptr[i + 1] - mptr[i]);
     }
     // transposed: for each member cell k, the (cell i, entry e) whose member is k, by i
     std::vector<int> tptr(n + 1, 0);
@@ -206,7 +207,8 @@ int iteration_vjp_core(tdt_ctx *c, const double *qn, bool nodes) {
         else
             HIPCALL(c, tdt::launch_add(c, (int)n5, a.g, b.lam), "adjoint: accumulate");
     }
-    HIPCALL(c, tdt::launch_op_jac(c, qn, b.mptr, b.mlist, b.JE, b.JEp, nodes ? b.JEx : nullptr),
+    HIPCALL(c, tdt::launch_op_jac(c, qn, b.mptr, b.mlist, b.kmax, b.JE, b.JEp,
+                                   nodes ? b.JEx : nullptr),
             "adjoint: operator blocks");
     HIPCALL(c, tdt::launch_op_vjp(c, b.JE, b.JEp, nodes ? b.JEx : nullptr, b.W, b.gdt, b.gvol,
                                   b.grtol, b.tptr, b.tlist, a.nptr, a.nlist, b.part, b.g, b.gp,
@@ -325,7 +327,7 @@ int tdt_operators_vjp(tdt_ctx *c, const double *cst, double rinit, const double 
         HIPCALL(c, zero(c, b.grtol, n), "clear");
         HIPCALL(c, zero(c, b.gp, 3), "clear");
         if (gxy) HIPCALL(c, zero(c, b.gxy, (size_t)2 * c->pm.m), "clear");
-        HIPCALL(c, tdt::launch_op_jac(c, b.qn, b.mptr, b.mlist, b.JE, b.JEp,
+        HIPCALL(c, tdt::launch_op_jac(c, b.qn, b.mptr, b.mlist, b.kmax, b.JE, b.JEp,
                                       gxy ? b.JEx : nullptr), "operator blocks");
         HIPCALL(c, tdt::launch_op_vjp(c, b.JE, b.JEp, gxy ? b.JEx : nullptr, b.W, b.gdt, b.gvol,
                                       b.grtol, b.tptr, b.tlist, c->adj->nptr, c->adj->nlist,

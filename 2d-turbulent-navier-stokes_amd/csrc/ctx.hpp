@@ -94,6 +94,7 @@ struct IterAdjBufs {
     int *mptr = nullptr, *mlist = nullptr;   // [n + 1], [members]
     int *tptr = nullptr;                     // [n + 1]
     int2 *tlist = nullptr;                   // (cell i, member entry e) by member cell
+    int kmax = 0;                            // largest member count
     bool xy_ok = true;                       // node derivatives available on this mesh
     double *JE = nullptr, *JEp = nullptr, *JEx = nullptr;
     double *qn = nullptr, *qs[3] = {nullptr, nullptr, nullptr}, *r[4] = {};
@@ -166,7 +167,7 @@ hipError_t launch_res_vjp(tdt_ctx *c, const double *qs, const double *v, const i
 // od: [D1..D5][n], ov: [VIS1..VIS4, DT][n]
 hipError_t launch_op_eval(tdt_ctx *c, const double *qs, double *od, double *ov);
 hipError_t launch_op_jac(tdt_ctx *c, const double *qs, const int *mptr, const int *mlist,
-                         double *JE, double *JEp, double *JEx);
+                         int kmax, double *JE, double *JEp, double *JEx);
 hipError_t launch_op_vjp(tdt_ctx *c, const double *JE, const double *JEp, const double *JEx,
                          const double *W, const double *gdt, const double *gvol,
                          const double *grtol, const int *tptr, const int2 *tlist,

@@ -28,7 +28,7 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "2d-turbulent-navier-stokes_amd"))
 
 import tdt  # noqa: E402
-from tdt import analysis, caseio, gridgen  # noqa: E402
+from tdt import analysis, caseio, gridgen, objective  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 DESCRIBE = {"channel_97x49": "2D turbulent channel / flat plate",
@@ -369,6 +369,114 @@ def batch_sweep(ncases=64, steps=100, warmup=20, seq_steps=50, precision="fp32-m
     return out
 
 
+ADJ_PARAMS = ("p0", "t0", "amachinit")
+
+
+def duct_j(mesh, flow, solver, niter):
+    """J of niter iterations from the cold start (tdt_iterate)."""
+    solver.cold_start()
+    solver.iterate(niter, history=False, check=False)
+    return objective.duct_objective(mesh, flow, solver.get_state())[0]
+
+
+def adjoint_study(niter=200, ref_iters=40, fd=True, reference=True):
+    """The discrete adjoint on the 193x65 duct with 8 Hicks-Henne bumps: J =
+    axial wall-pressure force + outlet mass flow after niter iterations from the
+    cold start; dJ/d(P0, T0, AMACHINIT) and dJ/d(bump amplitudes) from one
+    forward run (kept states) and one reverse sweep, against central
+    differences of tdt_iterate runs and, over ref_iters iterations, of the
+    oracle (the reference's arithmetic on the CPU)."""
+    import dataclasses
+    mk_mesh, mk_flow = gridgen.WORKLOADS["duct_193x65"]
+    mesh, flow = mk_mesh(), mk_flow()
+    duct_kw = dict(nx=193, ny=65, jitter=0.01 * 193 / 385)
+    dnode = objective.bump_node_derivatives(8, **duct_kw)
+    names = list(ADJ_PARAMS) + [f"bump{k + 1}" for k in range(8)]
+
+    def gradients(s, n):
+        t0 = time.perf_counter()
+        qn = s.adjoint_forward(n)
+        tf = time.perf_counter() - t0
+        j, djdq, djdxy = objective.duct_objective(mesh, flow, qn)
+        t0 = time.perf_counter()
+        _, gp, gxy = s.adjoint_reverse(djdq, nodes=True)
+        tr = time.perf_counter() - t0
+        dxy = gxy + djdxy
+        g = list(gp[:3]) + [float(np.sum(dxy * dnode[k])) for k in range(8)]
+        return j, dict(zip(names, g)), tf, tr
+
+    out = {"workload": f"duct_193x65: {DESCRIBE['duct_193x65']}, {mesh.neles} cells, "
+                       f"IAXIS=1, P0={flow.p0}, T0={flow.t0}, M={flow.amachinit}",
+           "objective": "J = axial wall-pressure force + outlet mass flow, per radian",
+           "iterations": niter}
+    with tdt.Solver(mesh, flow) as s:
+        j, grad, tf, tr = gradients(s, niter)
+        t0 = time.perf_counter()
+        duct_j(mesh, flow, s, niter)
+        s.synchronize()
+        t_iter = time.perf_counter() - t0
+        out.update({"J": j, "dJ": grad, "forward_with_checkpoints_s": round(tf, 3),
+                    "reverse_s": round(tr, 3), "tdt_iterate_s": round(t_iter, 4),
+                    "reverse_ms_per_iteration": round(tr * 1e3 / niter, 3)})
+        print(f"bench: adjoint J={j:.9e} forward {tf:.2f} s reverse {tr:.2f} s", file=sys.stderr,
+              flush=True)
+        if fd:
+            fdv = {}
+            for name in ADJ_PARAMS:
+                base = getattr(flow, name)
+                h = 1e-6 * base
+                jj = []
+                for sign in (1.0, -1.0):
+                    s.set_flow(dataclasses.replace(flow, **{name: base + sign * h}))
+                    jj.append(duct_j(mesh, flow, s, niter))
+                s.set_flow(flow)
+                fdv[name] = (jj[0] - jj[1]) / (2 * h)
+            for k in range(8):
+                h = 1e-3   # mm of amplitude
+                jj = []
+                for sign in (1.0, -1.0):
+                    x, y = mesh.x_mm + sign * h * dnode[k, 0], mesh.y_mm + sign * h * dnode[k, 1]
+                    s.set_nodes(x, y)
+                    jj.append(duct_j(dataclasses.replace(mesh, x_mm=x, y_mm=y), flow, s, niter))
+                s.set_nodes(mesh.x_mm, mesh.y_mm)
+                fdv[f"bump{k + 1}"] = (jj[0] - jj[1]) / (2 * h)
+            out["fd_tdt_iterate"] = fdv
+            out["rel_err_vs_fd"] = {k: float(f"{abs(grad[k] - fdv[k]) / max(abs(fdv[k]), 1e-300):.3e}")
+                                    for k in names}
+            print(f"bench: adjoint vs differences {out['rel_err_vs_fd']}", file=sys.stderr,
+                  flush=True)
+        if reference:
+            jr, gr, _, _ = gradients(s, ref_iters)
+    if reference:
+        # central differences of the oracle's J over ref_iters iterations
+        sys.path.insert(0, os.path.join(REPO, "oracle"))
+        from oracle import Oracle
+
+        def oracle_j(m, f):
+            o = Oracle(m, f)
+            o.cold_start()
+            o.iterate(ref_iters)
+            return objective.duct_objective(m, f, o.state())[0]
+        fdo = {}
+        for name in ("p0", "amachinit"):
+            base = getattr(flow, name)
+            h = 1e-6 * base
+            fdo[name] = (oracle_j(mesh, dataclasses.replace(flow, **{name: base + h}))
+                         - oracle_j(mesh, dataclasses.replace(flow, **{name: base - h}))) / (2 * h)
+        for k in (2, 5):
+            h = 1e-3
+            mp = dataclasses.replace(mesh, y_mm=mesh.y_mm + h * dnode[k, 1])
+            mm = dataclasses.replace(mesh, y_mm=mesh.y_mm - h * dnode[k, 1])
+            fdo[f"bump{k + 1}"] = (oracle_j(mp, flow) - oracle_j(mm, flow)) / (2 * h)
+        out["reference_check"] = {
+            "iterations": ref_iters, "adjoint": {k: gr[k] for k in fdo}, "fd_oracle": fdo,
+            "rel_err": {k: float(f"{abs(gr[k] - fdo[k]) / max(abs(fdo[k]), 1e-300):.3e}")
+                        for k in fdo}}
+        print(f"bench: adjoint vs oracle differences {out['reference_check']['rel_err']}",
+              file=sys.stderr, flush=True)
+    return out
+
+
 _SWEEP = []   # the CPU sweep's cases, generated before its workers fork
 
 
@@ -468,6 +576,12 @@ def main():
     ap.add_argument("--cpu-sweep", type=int, default=0, metavar="NCASES", help=argparse.SUPPRESS)
     ap.add_argument("--cpu-iters", type=int, default=20, help=argparse.SUPPRESS)
     ap.add_argument("--cpu-cores", type=int, default=0, help=argparse.SUPPRESS)
+    ap.add_argument("--adjoint", type=int, default=0, metavar="NITER",
+                    help="run the 193x65 duct adjoint study over NITER iterations, print its "
+                         "JSON line and exit")
+    ap.add_argument("--adjoint-ref-iters", type=int, default=40, help=argparse.SUPPRESS)
+    ap.add_argument("--no-fd", action="store_true",
+                    help="with --adjoint: skip the finite-difference checks")
     ap.add_argument("--no-sequential", action="store_true",
                     help="with --batch: skip the one-after-another runs")
     ap.add_argument("--no-large", action="store_true",
@@ -494,6 +608,12 @@ def main():
         avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
         cores = args.cpu_cores or min(16, avail or 1)
         print(json.dumps(cpu_sweep(args.cpu_sweep, args.cpu_iters, cores)), flush=True)
+        return
+    if args.adjoint:
+        out = adjoint_study(args.adjoint, ref_iters=args.adjoint_ref_iters,
+                            fd=not args.no_fd, reference=not args.no_cpu)
+        print(json.dumps({"metric": "discrete adjoint of the unrolled RK4 iteration on 1 MI355X",
+                          **out}), flush=True)
         return
     if args.batch:
         out = batch_sweep(args.batch, steps=args.steps if args.steps != 2000 else 100,

@@ -232,3 +232,37 @@ def test_wall_faces_of_each_workload():
         else:
             assert np.all(ymid <= 600.0 * np.tan(np.radians(15.0)) * 1.0001), name  # ramp top
             assert np.sum(ymid == 0.0) >= 97 * (name == "channel_97x49") + 819 * (name != "channel_97x49")
+
+
+def test_duct_objective_gradients():
+    """J = axial wall-pressure force + outlet mass flow of the 193x65 duct:
+    dJ/dc and the explicit dJ/dY against central differences, and the bump
+    node derivatives against a direct evaluation of the generator."""
+    import dataclasses
+    from tdt import gridgen, objective
+    mesh, flow = gridgen.WORKLOADS["duct_193x65"][0](), gridgen.WORKLOADS["duct_193x65"][1]()
+    rng = np.random.default_rng(2)
+    n = mesh.neles
+    c = np.stack([np.full(n, 1.2), np.full(n, 60.0), np.zeros(n), np.full(n, 2.6e5),
+                  np.full(n, 1.7e-5)]) * (1.0 + 0.01 * rng.standard_normal((5, n)))
+    j0, dc, dxy = objective.duct_objective(mesh, flow, c)
+    assert len(objective.analysis.wall_faces(mesh, flow.iaxis)) == 193
+    assert len(objective.outlet_faces(mesh)) == 65
+    w = c * rng.standard_normal(c.shape)
+    h = 1e-7
+    fd = (objective.duct_objective(mesh, flow, c + h * w)[0]
+          - objective.duct_objective(mesh, flow, c - h * w)[0]) / (2 * h)
+    assert abs(fd - np.sum(dc * w)) <= 1e-7 * np.sum(np.abs(dc * w))
+    wy = rng.standard_normal(mesh.nods) * 1e-2
+    jp = objective.duct_objective(dataclasses.replace(mesh, y_mm=mesh.y_mm + h * wy), flow, c)[0]
+    jm = objective.duct_objective(dataclasses.replace(mesh, y_mm=mesh.y_mm - h * wy), flow, c)[0]
+    assert abs((jp - jm) / (2 * h) - np.sum(dxy[1] * wy)) <= 1e-6 * np.sum(np.abs(dxy[1] * wy))
+    assert np.all(dxy[0] == 0.0)
+    kw = dict(nx=193, ny=65, jitter=0.01 * 193 / 385)
+    dn = objective.bump_node_derivatives(8, **kw)
+    assert dn.shape == (8, 2, mesh.nods) and np.all(dn[:, 0] == 0.0)
+    # the wall node under bump 3's centre moves by 1 per mm of amplitude
+    top = np.arange(65, mesh.nods, 66)
+    x3 = 50.0 + 300.0 * 3 / 7
+    col = np.argmin(np.abs(mesh.x_mm[top] - x3))
+    assert abs(dn[3, 1, top[col]] - gridgen.hicks_henne(mesh.x_mm[top[col]], [1.0], [x3])) < 1e-6
