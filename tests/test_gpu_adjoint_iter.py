@@ -448,3 +448,46 @@ def test_torch_unrolled_function(tmp_path):
                        timeout=240, cwd=REPO)
     assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
     assert "torch unrolled: OK" in r.stdout
+
+
+def test_duct_design_gradients():
+    """The 193x65 duct with 8 Hicks-Henne bumps: J = axial wall-pressure force +
+    outlet mass flow after N iterations from the cold start; dJ/dP0,
+    dJ/dAMACHINIT and dJ/d(bump amplitude) of two bumps from one reverse sweep
+    against central differences of tdt_iterate runs."""
+    import dataclasses
+    import tdt
+    from tdt import gridgen, objective
+    mesh, flow = gridgen.WORKLOADS["duct_193x65"][0](), gridgen.WORKLOADS["duct_193x65"][1]()
+    dnode = objective.bump_node_derivatives(8, nx=193, ny=65, jitter=0.01 * 193 / 385)
+    N = 30
+
+    def run_j(s, m, f):
+        s.cold_start()
+        s.iterate(N, history=False)
+        return objective.duct_objective(m, f, s.get_state())[0]
+
+    with tdt.Solver(mesh, flow) as s:
+        qn = s.adjoint_forward(N)
+        _, djdq, djdxy = objective.duct_objective(mesh, flow, qn)
+        _, gp, gxy = s.adjoint_reverse(djdq, nodes=True)
+        dxy = gxy + djdxy
+        for k, name in ((0, "p0"), (2, "amachinit")):
+            base = getattr(flow, name)
+            h = 1e-6 * base
+            j = []
+            for sign in (1.0, -1.0):
+                s.set_flow(dataclasses.replace(flow, **{name: base + sign * h}))
+                j.append(run_j(s, mesh, flow))
+            s.set_flow(flow)
+            check((j[0] - j[1]) / (2 * h), gp[k], abs(gp[k]), 1e-5, name)
+        for k in (1, 6):
+            ad = float(np.sum(dxy * dnode[k]))
+            h = 1e-3
+            j = []
+            for sign in (1.0, -1.0):
+                x, y = mesh.x_mm + sign * h * dnode[k, 0], mesh.y_mm + sign * h * dnode[k, 1]
+                s.set_nodes(x, y)
+                j.append(run_j(s, dataclasses.replace(mesh, x_mm=x, y_mm=y), flow))
+            s.set_nodes(mesh.x_mm, mesh.y_mm)
+            check((j[0] - j[1]) / (2 * h), ad, abs(ad), 1e-5, f"bump{k + 1}")
