@@ -30,7 +30,7 @@ namespace {
 
 constexpr int kJacBlocks = 5;         // stencil members per cell: self, faces 1..4
 constexpr int kJacStride = 5 * 5 * kJacBlocks;
-constexpr int kDualN = 5;             // directions per dual evaluation
+constexpr int kDualN = 1;             // directions per dual evaluation (per thread)
 // node slots per cell: the cell's four nodes, then each face neighbour's four
 // (a node shared by several slots collects the sum of their derivatives)
 constexpr int kNodeSlots = 20;
@@ -201,15 +201,19 @@ struct ParamSeeds {
     double ds0_dp0, ds0_dt0, dtinf_dt0, dblu_dt0;
 };
 
-// One dual evaluation per thread, blockIdx.y = e: member e < 5 (J[i] block e =
-// dR_i,m / dQ_s,c [m][c], zero for a ghost face), the parameters (e == 5: Jp[i] =
-// [m][p]: dR_i,m / d(P0, T0)) or node pass e - 6 (Jx[i], five of the 40
-// directions (slot, x or y)).
+constexpr int kMemberGroups = (5 + kDualN - 1) / kDualN;             // per member
+constexpr int kParGroups = (2 + kDualN - 1) / kDualN;                // P0, T0
+constexpr int kNodeGroups = (2 * kNodeSlots + kDualN - 1) / kDualN;  // slot x, y
+
+// One dual evaluation per thread, blockIdx.y = e: member s = e / kMemberGroups
+// (s < 5) with components c0.. seeded (J[i] block s = dR_i,m / dQ_s,c [m][c],
+// zero for a ghost face); then the parameter groups (Jp[i] = [m][p]: dR_i,m /
+// d(P0, T0)); then the node groups (Jx[i], directions (slot, x or y)).
 __global__ void __launch_bounds__(kBlock) k_res_jac(Dev D, Consts<double> K, ParamSeeds ps,
                                                     const double *qs, double *J, double *Jp,
                                                     double *Jx) {
     const int i = gid();
-    const int e = blockIdx.y;
+    int e = blockIdx.y;
     if (i >= D.n) return;
     const int n = D.n;
     const int4 nb4 = D.nc4[i];
@@ -220,46 +224,52 @@ __global__ void __launch_bounds__(kBlock) k_res_jac(Dev D, Consts<double> K, Par
     Dual<kDualN> q[5][5], r[5];
     for (int a = 0; a < 5; ++a)
         for (int c = 0; c < 5; ++c) q[a][c] = Dual<kDualN>(qd[a][c]);
-    if (e < kJacBlocks) {
-        double *Ji = J + (size_t)i * kJacStride + e * 25;
-        if (e > 0 && nb[e - 1] >= n) {
-            for (int k = 0; k < 25; ++k) Ji[k] = 0.0;
+    if (e < kJacBlocks * kMemberGroups) {
+        const int sm = e / kMemberGroups, c0 = (e % kMemberGroups) * kDualN;
+        double *Ji = J + (size_t)i * kJacStride + sm * 25;
+        if (sm > 0 && nb[sm - 1] >= n) {
+            for (int k = 0; k < kDualN && c0 + k < 5; ++k)
+                for (int m = 0; m < 5; ++m) Ji[m * 5 + c0 + k] = 0.0;
             return;
         }
         CellGeo<Dual<kDualN>> GD;
         geo_stored(D, i, GD);
-        for (int c = 0; c < 5; ++c) q[e][c].d[c] = 1.0;   // direction c: Q_e,c
+        for (int k = 0; k < kDualN && c0 + k < 5; ++k) q[sm][c0 + k].d[k] = 1.0;
         cell_residual<Dual<kDualN>>(D, i, q, GD, KD, r);
-        for (int m = 0; m < 5; ++m)
-            for (int c = 0; c < 5; ++c) Ji[m * 5 + c] = r[m].d[c];
+        for (int k = 0; k < kDualN && c0 + k < 5; ++k)
+            for (int m = 0; m < 5; ++m) Ji[m * 5 + c0 + k] = r[m].d[k];
         return;
     }
-    if (e == kJacBlocks) {   // parameters: directions 0 (P0) and 1 (T0)
+    e -= kJacBlocks * kMemberGroups;
+    if (e < kParGroups) {   // parameter p = e * kDualN + k in direction k
         Consts<Dual<kDualN>> KP = KD;
-        KP.s0.d[0] = ps.ds0_dp0;
-        KP.s0.d[1] = ps.ds0_dt0;
-        KP.t0.d[1] = 1.0;
-        KP.tinf.d[1] = ps.dtinf_dt0;
-        KP.bl_u.d[1] = ps.dblu_dt0;
+        for (int k = 0; k < kDualN; ++k) {
+            const int p = e * kDualN + k;
+            if (p == 0) KP.s0.d[k] = ps.ds0_dp0;
+            if (p == 1) {
+                KP.s0.d[k] = ps.ds0_dt0;
+                KP.t0.d[k] = 1.0;
+                KP.tinf.d[k] = ps.dtinf_dt0;
+                KP.bl_u.d[k] = ps.dblu_dt0;
+            }
+        }
         CellGeo<Dual<kDualN>> GD;
         geo_stored(D, i, GD);
         cell_residual<Dual<kDualN>>(D, i, q, GD, KP, r);
-        for (int m = 0; m < 5; ++m) {
-            Jp[(size_t)i * 10 + m * 2] = r[m].d[0];
-            Jp[(size_t)i * 10 + m * 2 + 1] = r[m].d[1];
-        }
+        for (int k = 0; k < kDualN && e * kDualN + k < 2; ++k)
+            for (int m = 0; m < 5; ++m) Jp[(size_t)i * 10 + m * 2 + e * kDualN + k] = r[m].d[k];
         return;
     }
-    if (!Jx) return;
-    const int d0 = (e - kJacBlocks - 1) * kDualN;   // node coordinates: this pass's directions
-    if (d0 >= 2 * kNodeSlots) return;
+    e -= kParGroups;
+    if (!Jx || e >= kNodeGroups) return;
+    const int d0 = e * kDualN;   // node coordinates: this group's directions
     Dual<kDualN> x[kNodeSlots], y[kNodeSlots];
     for (int sl = 0; sl < kNodeSlots; ++sl) {
         const int v = slot_node(D, i, sl);
         x[sl] = Dual<kDualN>(v >= 0 ? D.xn[v] : 0.0);
         y[sl] = Dual<kDualN>(v >= 0 ? D.yn[v] : 0.0);
     }
-    for (int k = 0; k < kDualN; ++k) {   // direction k: slot (d0+k)/2, x if even
+    for (int k = 0; k < kDualN && d0 + k < 2 * kNodeSlots; ++k) {   // slot (d0+k)/2, x if even
         const int dir = d0 + k, sl = dir >> 1;
         if (dir & 1) y[sl].d[k] = 1.0;
         else x[sl].d[k] = 1.0;
@@ -268,7 +278,7 @@ __global__ void __launch_bounds__(kBlock) k_res_jac(Dev D, Consts<double> K, Par
     geo_from_nodes(D, i, x, y, K.iaxis, GX);
     cell_residual<Dual<kDualN>>(D, i, q, GX, KD, r);
     double *Jxi = Jx + (size_t)i * kXyStride;
-    for (int k = 0; k < kDualN; ++k)
+    for (int k = 0; k < kDualN && d0 + k < 2 * kNodeSlots; ++k)
         for (int m = 0; m < 5; ++m) Jxi[(d0 + k) * 5 + m] = r[m].d[k];
 }
 
@@ -375,7 +385,7 @@ hipError_t launch_res_vjp(tdt_ctx *c, const double *qs, const double *v, const i
     ps.dtinf_dt0 = (double)K.tinf / P.t0;
     ps.dblu_dt0 = 0.5 * (double)K.bl_u / P.t0;
     const unsigned nb = nblocks(c->D.n);
-    const unsigned evals = kJacBlocks + 1 + (gxy ? 2 * kNodeSlots / kDualN : 0);
+    const unsigned evals = kJacBlocks * kMemberGroups + kParGroups + (gxy ? kNodeGroups : 0);
     hipLaunchKernelGGL(k_res_jac, dim3(nb, evals), dim3(kBlock), 0, c->stream, c->D, K, ps, qs,
                        J, Jp, gxy ? Jx : nullptr);
     hipLaunchKernelGGL(k_res_vjp, dim3(nb), dim3(kBlock), 0, c->stream, c->D, J, v, tptr, tlist,

@@ -33,7 +33,8 @@ namespace {
 
 using std::fabs;   // FP64 operands; Dual ones find tdt::fabs by argument lookup
 
-constexpr int kDualN = 5;
+// directions per dual evaluation: fewer directions per thread, more threads
+constexpr int kDualN = 1;
 constexpr int kOpRows = 6;                 // member block rows: E1..E5, DT
 constexpr int kOpBlock = kOpRows * 5;      // [row][c]
 constexpr int kOpParRows = 5;              // E1..E5 x (P0, T0, RTOL)
@@ -365,14 +366,14 @@ struct ValueLoader {
     }
 };
 
-// member `seed` carries the five directions dQ_seed,c
+// member `seed` carries directions k = 0..kDualN-1: dQ_seed,(c0 + k)
 struct SeedLoader {
     const double *qs;
-    int n, seed;
+    int n, seed, c0;
     __device__ void operator()(int id, Dual<kDualN> (&q)[5]) const {
         for (int c = 0; c < 5; ++c) {
             q[c] = Dual<kDualN>(qs[(size_t)c * n + id]);
-            if (id == seed) q[c].d[c] = 1.0;
+            if (id == seed && c >= c0 && c < c0 + kDualN) q[c].d[c - c0] = 1.0;
         }
     }
 };
@@ -406,56 +407,71 @@ struct OpSeeds {
     double ds0_dp0, ds0_dt0, dtinf_dt0, dblu_dt0, rtred;
 };
 
-// One dual evaluation per thread: blockIdx.y = e selects, for cell i, member e
-// (e < its member count: JE[mptr[i] + e] = d(E_i, DT_i)/dQ_(member) [row][c]),
-// the parameters (e == kmax: JEp[i] = dE_i/d(P0, T0, RTOL) [row][p]) or node
-// pass e - kmax - 1 (JEx[i], d(E_i, DT_i, VOL_i)/d(slot x, y) for the five
-// directions of that pass).  kmax = the largest member count.
+constexpr int kMemberGroups = (5 + kDualN - 1) / kDualN;             // per member
+constexpr int kParGroups = (3 + kDualN - 1) / kDualN;                // P0, T0, RTOL
+constexpr int kNodeGroups = (2 * kNodeSlots + kDualN - 1) / kDualN;  // slot x, y
+
+// One dual evaluation per thread, blockIdx.y = e: for cell i, member m = e /
+// kMemberGroups (m < its member count) with components c0.. of that member seeded
+// (JE[mptr[i] + m] = d(E_i, DT_i)/dQ_member [row][c]); then the parameter groups
+// (JEp[i] = dE_i/d(P0, T0, RTOL) [row][p]); then the node groups (JEx[i] =
+// d(E_i, DT_i, VOL_i)/d(slot x, y)).  kmax = the largest member count.
 __global__ void __launch_bounds__(64) k_op_jac(Dev D, Consts<double> K, OpSeeds ps,
                                                const double *qs, const int *mptr,
                                                const int *mlist, int kmax, double *JE,
                                                double *JEp, double *JEx) {
     const int i = blockIdx.x * 64 + threadIdx.x;
-    const int e = blockIdx.y;
+    int e = blockIdx.y;
     if (i >= D.n) return;
     const int n = D.n;
     const Consts<Dual<kDualN>> KD = lift<kDualN>(K);
-    if (e < kmax) {
-        const int ent = mptr[i] + e;
+    if (e < kmax * kMemberGroups) {
+        const int ent = mptr[i] + e / kMemberGroups;
+        const int c0 = (e % kMemberGroups) * kDualN;
         if (ent >= mptr[i + 1]) return;
         OpGeoStored<Dual<kDualN>> G{&D, i};
         OpOut<Dual<kDualN>> o;
-        cell_ops<Dual<kDualN>>(D, i, SeedLoader{qs, n, mlist[ent]}, G, KD, o);
+        cell_ops<Dual<kDualN>>(D, i, SeedLoader{qs, n, mlist[ent], c0}, G, KD, o);
         double *B = JE + (size_t)ent * kOpBlock;
-        for (int c = 0; c < 5; ++c) {
-            B[c] = o.d[0].d[c];
-            for (int m = 1; m < 5; ++m) B[m * 5 + c] = o.d[m].d[c] + o.vis[m - 1].d[c];
-            B[25 + c] = o.dt.d[c];
+        for (int k = 0; k < kDualN && c0 + k < 5; ++k) {
+            const int c = c0 + k;
+            B[c] = o.d[0].d[k];
+            for (int m = 1; m < 5; ++m) B[m * 5 + c] = o.d[m].d[k] + o.vis[m - 1].d[k];
+            B[25 + c] = o.dt.d[k];
         }
         return;
     }
-    if (e == kmax) {   // parameters: directions 0 (P0), 1 (T0), 2 (RTOL)
+    e -= kmax * kMemberGroups;
+    if (e < kParGroups) {   // parameter p = e * kDualN + k in direction k
         Consts<Dual<kDualN>> KP = KD;
-        KP.s0.d[0] = ps.ds0_dp0;
-        KP.s0.d[1] = ps.ds0_dt0;
-        KP.t0.d[1] = 1.0;
-        KP.tinf.d[1] = ps.dtinf_dt0;
-        KP.bl_u.d[1] = ps.dblu_dt0;
-        KP.rtol.d[2] = 1.0;
-        KP.rtol_rtred.d[2] = ps.rtred;
+        for (int k = 0; k < kDualN; ++k) {
+            const int p = e * kDualN + k;
+            if (p == 0) KP.s0.d[k] = ps.ds0_dp0;
+            if (p == 1) {
+                KP.s0.d[k] = ps.ds0_dt0;
+                KP.t0.d[k] = 1.0;
+                KP.tinf.d[k] = ps.dtinf_dt0;
+                KP.bl_u.d[k] = ps.dblu_dt0;
+            }
+            if (p == 2) {
+                KP.rtol.d[k] = 1.0;
+                KP.rtol_rtred.d[k] = ps.rtred;
+            }
+        }
         OpGeoStored<Dual<kDualN>> G{&D, i};
         OpOut<Dual<kDualN>> o;
         cell_ops<Dual<kDualN>>(D, i, ConstLoader{qs, n}, G, KP, o);
         double *P = JEp + (size_t)i * kOpParRows * 3;
-        for (int p = 0; p < 3; ++p) {
-            P[p] = o.d[0].d[p];
-            for (int m = 1; m < 5; ++m) P[m * 3 + p] = o.d[m].d[p] + o.vis[m - 1].d[p];
+        for (int k = 0; k < kDualN && e * kDualN + k < 3; ++k) {
+            const int p = e * kDualN + k;
+            P[p] = o.d[0].d[k];
+            for (int m = 1; m < 5; ++m) P[m * 3 + p] = o.d[m].d[k] + o.vis[m - 1].d[k];
         }
         return;
     }
-    if (!JEx) return;
-    const int d0 = (e - kmax - 1) * kDualN;   // first of this pass's directions
-    if (d0 >= 2 * kNodeSlots) return;
+    e -= kParGroups;
+    if (!JEx || e >= kNodeGroups) return;
+    const int d0 = e * kDualN;   // first of this group's directions
     OpGeoSlots<Dual<kDualN>> GS;
     GS.D = &D;
     GS.i = i;
@@ -474,7 +490,7 @@ __global__ void __launch_bounds__(64) k_op_jac(Dev D, Consts<double> K, OpSeeds 
         GS.xs[sl] = Dual<kDualN>(v >= 0 ? D.xn[v] : 0.0);
         GS.ys[sl] = Dual<kDualN>(v >= 0 ? D.yn[v] : 0.0);
     }
-    for (int k = 0; k < kDualN; ++k) {   // direction k: slot (d0+k)/2, x if even
+    for (int k = 0; k < kDualN && d0 + k < 2 * kNodeSlots; ++k) {   // slot (d0+k)/2, x if even
         const int dir = d0 + k, sl = dir >> 1;
         if (dir & 1) GS.ys[sl].d[k] = 1.0;
         else GS.xs[sl].d[k] = 1.0;
@@ -483,7 +499,7 @@ __global__ void __launch_bounds__(64) k_op_jac(Dev D, Consts<double> K, OpSeeds 
     OpOut<Dual<kDualN>> o;
     cell_ops<Dual<kDualN>>(D, i, ConstLoader{qs, n}, GS, KD, o);
     double *X = JEx + (size_t)i * kOpXyStride;
-    for (int k = 0; k < kDualN; ++k) {
+    for (int k = 0; k < kDualN && d0 + k < 2 * kNodeSlots; ++k) {
         double *Xr = X + (size_t)(d0 + k) * kOpXyRows;
         Xr[0] = o.d[0].d[k];
         for (int m = 1; m < 5; ++m) Xr[m] = o.d[m].d[k] + o.vis[m - 1].d[k];
@@ -669,7 +685,7 @@ hipError_t launch_op_jac(tdt_ctx *c, const double *qs, const int *mptr, const in
     ps.dtinf_dt0 = (double)K.tinf / P.t0;
     ps.dblu_dt0 = 0.5 * (double)K.bl_u / P.t0;
     ps.rtred = c->params_rtred;
-    const unsigned evals = kmax + 1 + (JEx ? 2 * kNodeSlots / kDualN : 0);
+    const unsigned evals = kmax * kMemberGroups + kParGroups + (JEx ? kNodeGroups : 0);
     hipLaunchKernelGGL(k_op_jac, dim3((c->D.n + 63) / 64, evals), dim3(64), 0, c->stream, c->D,
                        K, ps, qs, mptr, mlist, kmax, JE, JEp, JEx);
     return hipGetLastError();
