@@ -142,6 +142,7 @@ int tdt_set_params(tdt_ctx *c, const tdt_params *P) {
     c->Kf = make_consts<float>(*P);
     set_rinit(c, c->rinit);
     tdt::fused_free(c);   // its graphs hold the old constants; rebuilt on the next iterate
+    if (c->iadj) c->iadj->niter = 0;   // a kept forward run used the old constants
     return TDT_OK;
 }
 
@@ -164,6 +165,7 @@ int tdt_set_nodes(tdt_ctx *c, const double *x_mm, const double *y_mm) {
             "node upload");
     HIPCALL(c, tdt::launch_geometry(c, c->params.iaxis), "geometry");
     ++c->geom_version;
+    if (c->iadj) c->iadj->niter = 0;   // a kept forward run used the old geometry
     return sync(c);
 }
 

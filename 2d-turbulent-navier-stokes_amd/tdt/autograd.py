@@ -34,17 +34,23 @@ lib.load()
 import torch  # noqa: E402  (after the library, see above)
 
 
-def _use_params(solver, p) -> None:
+def _use_params(solver, p) -> bool:
+    """Sets the solver's P0, T0, AMACHINIT from p; True if they changed."""
     p0, t0, mach = (float(x) for x in p.detach().cpu().reshape(3))
     f = solver.flow
     if (f.p0, f.t0, f.amachinit) != (p0, t0, mach):
         solver.set_flow(dataclasses.replace(f, p0=p0, t0=t0, amachinit=mach))
+        return True
+    return False
 
 
-def _use_nodes(solver, xy) -> None:
+def _use_nodes(solver, xy) -> bool:
+    """Sets the solver's node coordinates from xy; True if they changed."""
     a = xy.detach().cpu().numpy().astype(np.float64).reshape(2, -1)
     if not np.array_equal(a, solver.xy_mm):
         solver.set_nodes(a[0], a[1])
+        return True
+    return False
 
 
 class Residual(torch.autograd.Function):
@@ -99,9 +105,9 @@ class Unrolled(torch.autograd.Function):
     def backward(ctx, grad_qn):
         p, xy = ctx.saved_tensors
         solver = ctx.solver
-        _use_params(solver, p)
-        _use_nodes(solver, xy)
-        if getattr(solver, "_adjoint_run", None) != ctx.run:   # another run came since
+        changed = _use_params(solver, p)
+        changed = _use_nodes(solver, xy) or changed   # either discards the kept run
+        if changed or getattr(solver, "_adjoint_run", None) != ctx.run:   # not this run's states
             solver.adjoint_forward(ctx.niter)
             solver._adjoint_run = ctx.run
         _, gp, gxy = solver.adjoint_reverse(grad_qn.detach().cpu().numpy().astype(np.float64),

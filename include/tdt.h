@@ -185,7 +185,9 @@ int tdt_adjoint_forward(tdt_ctx *ctx, int32_t niter, const double *c0, double ri
  * the cold start computes from them) and the fourth is the part through RINIT
  * alone; from a given c0 they hold Q^0 and RINIT fixed (AMACHINIT: 0) --; gxy
  * ([2][nods] or NULL) = dJ/d(X, Y) per grid.in millimetre.  The stage states of
- * each iteration are recomputed from the kept states, not stored. */
+ * each iteration are recomputed from the kept states, not stored.  tdt_set_params
+ * and tdt_set_nodes discard the kept run (TDT_ERR_UNSUPPORTED until the next
+ * tdt_adjoint_forward). */
 int tdt_adjoint_reverse(tdt_ctx *ctx, const double *dj_dcn, double *lam0, double *gparam,
                         double *gxy);
 

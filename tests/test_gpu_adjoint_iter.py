@@ -434,6 +434,20 @@ with tdt.Solver(mesh, flow) as s:
     _, gp, gxy = s.adjoint_reverse(v, nodes=True)
     assert np.array_equal(p.grad.numpy(), gp[:3]), "grad p"
     assert np.array_equal(xy.grad.numpy(), gxy), "grad xy"
+    # a backward after other parameters were set runs its own forward again
+    q2 = unrolled(p, s, N, xy)
+    s.set_flow(flow.__class__(**{**flow.__dict__, "p0": 1.1 * flow.p0}))
+    p.grad = None
+    (q2 * torch.from_numpy(v)).sum().backward()
+    assert np.array_equal(p.grad.numpy(), gp[:3]), "grad p after set_flow"
+    # the C call refuses a reverse once the parameters changed under it
+    s.adjoint_forward(N)
+    s.set_flow(flow)
+    try:
+        s.adjoint_reverse(v)
+        raise SystemExit("reverse after set_flow was not refused")
+    except tdt.TdtError as exc:
+        assert exc.code == tdt.lib.TDT_ERR_UNSUPPORTED
 print("torch unrolled: OK", flush=True)
 """
 
