@@ -590,6 +590,8 @@ def main():
                     help="run the 193x65 duct adjoint study over NITER iterations, print its "
                          "JSON line and exit")
     ap.add_argument("--adjoint-ref-iters", type=int, default=40, help=argparse.SUPPRESS)
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the 200-iteration duct adjoint study beside the main workload")
     ap.add_argument("--no-fd", action="store_true",
                     help="with --adjoint: skip the finite-difference checks")
     ap.add_argument("--no-sequential", action="store_true",
@@ -697,6 +699,12 @@ def main():
         large_case(stream, converge=0 if args.converge == 0 else 20000, cpu=not args.no_cpu)
     small = None if args.no_small or args.workload == "channel_97x49" else \
         small_case(stream, converge=0 if args.converge == 0 else 2000, cpu=not args.no_cpu)
+    adjoint = None
+    if not args.no_adjoint:
+        try:
+            adjoint = adjoint_study(200, ref_iters=40, fd=True, reference=not args.no_cpu)
+        except tdt.TdtError as exc:
+            adjoint = {"error": str(exc)}
     line = {
         "metric": "Mcell-iters/s on 1 MI355X, FP64 vs FP32-mixed at fixed error; % HBM peak",
         "value": round(v64, 2), "unit": "Mcell-iters/s", "n_gpus": 1, "steps": args.steps,
@@ -711,6 +719,7 @@ def main():
         "roofline": roof,
         "large_case": large,
         "small_case": small,
+        "adjoint": adjoint,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
