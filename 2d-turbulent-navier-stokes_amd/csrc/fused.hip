@@ -236,6 +236,7 @@ void fused_free(tdt_ctx *c) {
     if (B->have_events)
         for (int k = 0; k < 12; ++k) hipEventDestroy(B->ev[k]);
     for (void *p : B->own) hipFree(p);
+    if (B->hist_host) hipHostFree(B->hist_host);
     delete B;
     c->fused = nullptr;
 }
@@ -386,14 +387,35 @@ int fused_iterate(tdt_ctx *c, int niter, double *l2, double *linf, int *first_no
     }
     const size_t nrec = F.ncases ? (size_t)F.ncases : 1;   // history records per iteration
     if ((size_t)niter > B.hist_cap) {
-        size_t cap = std::max((size_t)niter, (size_t)1024);
+        // room for 65536 iterations of one case (5 MB) from the first call, so
+        // runs of usual lengths never reallocate it and recapture their graphs
+        size_t cap = std::max((size_t)niter, std::max((size_t)1024, (size_t)65536 / nrec));
         double *h = dalloc<double>(B, cap * 10 * nrec);
         if (!h) {
             c->err = "device allocation of the monitor history failed";
             return TDT_ERR_DEVICE;
         }
+        // the history is read back into pinned memory: a pageable copy of this
+        // size pays a one-time runtime staging cost of several milliseconds
+        double *hh = nullptr;
+        if (hipHostMalloc((void **)&hh, cap * 10 * nrec * sizeof(double), 0) != hipSuccess) {
+            c->err = "pinned allocation of the monitor history failed";
+            return TDT_ERR_DEVICE;
+        }
+        if (B.hist_host) (void)hipHostFree(B.hist_host);
+        B.hist_host = hh;
         F.hist = h;
         B.hist_cap = cap;
+        // one copy of the whole buffer now: the runtime sets up its path for
+        // large device-to-host copies on first use (several milliseconds), which
+        // would otherwise land in the first run that reads a long history back
+        hipError_t ec = hipMemcpyAsync(hh, h, cap * 10 * nrec * sizeof(double),
+                                       hipMemcpyDeviceToHost, c->stream);
+        if (ec == hipSuccess) ec = hipStreamSynchronize(c->stream);
+        if (ec != hipSuccess) {
+            c->err = std::string("monitor history: ") + hipGetErrorString(ec);
+            return TDT_ERR_DEVICE;
+        }
         // kernels captured earlier point at the old history buffer
         if (B.exec1) { hipGraphExecDestroy(B.exec1); B.exec1 = nullptr; }
         if (B.execK) { hipGraphExecDestroy(B.execK); B.execK = nullptr; }
@@ -452,19 +474,18 @@ int fused_iterate(tdt_ctx *c, int niter, double *l2, double *linf, int *first_no
     if (e == hipSuccess) e = hipGetLastError();
     int ctr[4] = {0, 0, 0, 0};
     if (e == hipSuccess) e = hipMemcpyAsync(ctr, F.ctr, sizeof ctr, hipMemcpyDeviceToHost, c->stream);
-    std::vector<double> h;
-    if (e == hipSuccess && (l2 || linf) && niter > 0) {
-        h.resize((size_t)niter * 10 * nrec);
-        e = hipMemcpyAsync(h.data(), F.hist, h.size() * sizeof(double), hipMemcpyDeviceToHost,
+    const size_t nvals = (l2 || linf) && niter > 0 ? (size_t)niter * 10 * nrec : 0;
+    if (e == hipSuccess && nvals)
+        e = hipMemcpyAsync(B.hist_host, F.hist, nvals * sizeof(double), hipMemcpyDeviceToHost,
                            c->stream);
-    }
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {
         c->err = std::string("fused iteration: ") + hipGetErrorString(e);
         return TDT_ERR_DEVICE;
     }
     // records [iteration][case]: L2 in 0..4, Linf in 5..9
-    for (size_t r = 0; r < h.size() / 10; ++r)
+    const double *h = B.hist_host;
+    for (size_t r = 0; r < nvals / 10; ++r)
         for (int q = 0; q < 5; ++q) {
             if (l2) l2[r * 5 + q] = h[r * 10 + q];
             if (linf) linf[r * 5 + q] = h[r * 10 + 5 + q];

@@ -72,6 +72,7 @@ struct FusedBufs {
     FDev F{};
     int nb_cells = 0, nb_nodes = 0;
     size_t hist_cap = 0;
+    double *hist_host = nullptr;    // pinned copy target of the history, hist_cap records
     std::vector<void *> own;
     hipGraph_t graph1 = nullptr, graphK = nullptr;
     hipGraphExec_t exec1 = nullptr, execK = nullptr;

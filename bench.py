@@ -111,8 +111,8 @@ def roofline(workload, precision, mesh, flow, kt, kset, elapsed, steps):
 def run_gpu(mesh, flow, precision, steps, warmup, ordering="auto"):
     with tdt.Solver(mesh, flow, precision, ordering) as s:
         s.cold_start()
-        if warmup:
-            s.iterate(warmup, history=False)
+        if warmup:   # the same call as the timed one (history read back)
+            s.iterate(warmup)
         s.synchronize()
         t0 = time.perf_counter()
         l2, _, bad = s.iterate(steps)      # returns after its stream has drained
