@@ -13,7 +13,6 @@ from __future__ import annotations
 import ctypes
 import os
 import re
-from typing import Optional
 
 import numpy as np
 

@@ -13,7 +13,6 @@ states (developed flows; at a uniform cold start the turbulence production is
 exactly 0 and its square root has no derivative).
 """
 import dataclasses
-import os
 import subprocess
 import sys
 

@@ -1,7 +1,6 @@
 """Host-side tests that need no GPU: the C ABI library, case I/O, grid
 generation and mesh validation."""
 import dataclasses
-import os
 
 import numpy as np
 import pytest
