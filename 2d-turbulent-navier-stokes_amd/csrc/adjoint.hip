@@ -372,10 +372,7 @@ hipError_t launch_res_eval(tdt_ctx *c, const double *qs, double *r) {
     return hipGetLastError();
 }
 
-hipError_t launch_res_vjp(tdt_ctx *c, const double *qs, const double *v, const int *tptr,
-                          const int2 *tlist, double *J, double *Jp, double *part, double *g,
-                          double *gp, double *Jx, const int *nptr, const int2 *nlist,
-                          double *gxy) {
+static ParamSeeds param_seeds(const tdt_ctx *c) {
     const Consts<double> &K = c->K;
     const tdt_params &P = c->params;
     ParamSeeds ps;
@@ -384,18 +381,38 @@ hipError_t launch_res_vjp(tdt_ctx *c, const double *qs, const double *v, const i
     ps.ds0_dt0 = gam * (double)K.s0 / P.t0;
     ps.dtinf_dt0 = (double)K.tinf / P.t0;
     ps.dblu_dt0 = 0.5 * (double)K.bl_u / P.t0;
+    return ps;
+}
+
+hipError_t launch_res_jac(tdt_ctx *c, const double *qs, double *J, double *Jp, double *Jx) {
     const unsigned nb = nblocks(c->D.n);
-    const unsigned evals = kJacBlocks * kMemberGroups + kParGroups + (gxy ? kNodeGroups : 0);
-    hipLaunchKernelGGL(k_res_jac, dim3(nb, evals), dim3(kBlock), 0, c->stream, c->D, K, ps, qs,
-                       J, Jp, gxy ? Jx : nullptr);
+    const unsigned evals = kJacBlocks * kMemberGroups + kParGroups + (Jx ? kNodeGroups : 0);
+    hipLaunchKernelGGL(k_res_jac, dim3(nb, evals), dim3(kBlock), 0, c->stream, c->D, c->K,
+                       param_seeds(c), qs, J, Jp, Jx);
+    return hipGetLastError();
+}
+
+hipError_t launch_res_apply(tdt_ctx *c, const double *J, const double *Jp, const double *v,
+                            const int *tptr, const int2 *tlist, double *part, double *g,
+                            double *gp) {
+    const unsigned nb = nblocks(c->D.n);
     hipLaunchKernelGGL(k_res_vjp, dim3(nb), dim3(kBlock), 0, c->stream, c->D, J, v, tptr, tlist,
                        g);
     hipLaunchKernelGGL(k_par_vjp, dim3(nb), dim3(kBlock), 0, c->stream, c->D, Jp, v, part);
     hipLaunchKernelGGL(k_par_fold, dim3(1), dim3(kWave), 0, c->stream, part, (int)nb, gp);
-    if (gxy)
+    return hipGetLastError();
+}
+
+hipError_t launch_res_vjp(tdt_ctx *c, const double *qs, const double *v, const int *tptr,
+                          const int2 *tlist, double *J, double *Jp, double *part, double *g,
+                          double *gp, double *Jx, const int *nptr, const int2 *nlist,
+                          double *gxy) {
+    hipError_t e = launch_res_jac(c, qs, J, Jp, gxy ? Jx : nullptr);
+    if (e == hipSuccess) e = launch_res_apply(c, J, Jp, v, tptr, tlist, part, g, gp);
+    if (e == hipSuccess && gxy)
         hipLaunchKernelGGL(k_xy_vjp, dim3(nblocks(c->D.m)), dim3(kBlock), 0, c->stream, c->D, Jx,
                            v, nptr, nlist, gxy);
-    return hipGetLastError();
+    return e == hipSuccess ? hipGetLastError() : e;
 }
 
 }  // namespace tdt

@@ -218,6 +218,68 @@ int iteration_vjp_core(tdt_ctx *c, const double *qn, bool nodes) {
     return TDT_OK;
 }
 
+// Fixed-point adjoint at the state qn: the iteration recomputed once (stage
+// states, residuals, D, VIS, DT kept in the buffers and routine arrays) and the
+// residual blocks at the four stage states and the operator blocks formed once.
+int fixed_point_prepare(tdt_ctx *c, const double *qn) {
+    tdt::IterAdjBufs &b = *c->iadj;
+    const int n = c->pm.n;
+    for (int k = 0; k < 4; ++k) {
+        if (!b.Jst[k]) b.Jst[k] = dev_alloc<double>(c, (size_t)125 * n, false);
+        if (!b.Jpst[k]) b.Jpst[k] = dev_alloc<double>(c, (size_t)10 * n, false);
+        if (!b.Jst[k] || !b.Jpst[k]) {
+            c->err = "fixed-point adjoint: device allocation failed";
+            return TDT_ERR_DEVICE;
+        }
+    }
+    if (!b.gJ) b.gJ = dev_alloc<double>(c, (size_t)5 * n);
+    if (!b.gpx) b.gpx = dev_alloc<double>(c, 3);
+    if (!b.gJ || !b.gpx) {
+        c->err = "fixed-point adjoint: device allocation failed";
+        return TDT_ERR_DEVICE;
+    }
+    HIPCALL(c, tdt::launch_unpack(c, qn), "adjoint: state");
+    HIPCALL(c, routine_iteration(c, true), "adjoint: forward recomputation");
+    for (int k = 0; k < 4; ++k)
+        HIPCALL(c, tdt::launch_res_jac(c, k == 0 ? qn : b.qs[k - 1], b.Jst[k], b.Jpst[k], nullptr),
+                "adjoint: residual blocks");
+    HIPCALL(c, tdt::launch_op_jac(c, qn, b.mptr, b.mlist, b.kmax, b.JE, b.JEp, nullptr),
+            "adjoint: operator blocks");
+    return TDT_OK;
+}
+
+// b.mu -> b.lam = Phi_Q^T mu with the blocks fixed_point_prepare kept (no
+// parameter or node derivatives)
+int frozen_vjp(tdt_ctx *c, const double *qn) {
+    tdt::IterAdjBufs &b = *c->iadj;
+    tdt::AdjBufs &a = *c->adj;
+    const tdt::Dev &D = c->D;
+    const int n = D.n;
+    const size_t n5 = (size_t)5 * n;
+    HIPCALL(c, zero(c, b.lam, n5), "adjoint: clear");
+    HIPCALL(c, zero(c, b.W, n5), "adjoint: clear");
+    HIPCALL(c, zero(c, b.gdt, n), "adjoint: clear");
+    HIPCALL(c, zero(c, b.gvol, n), "adjoint: clear");
+    HIPCALL(c, zero(c, b.grtol, n), "adjoint: clear");
+    for (int k = 3; k >= 0; --k) {
+        HIPCALL(c, tdt::launch_stage_vjp(c, kAlphas[k], qn, b.r[k], D.vis, D.d24, D.dt, D.vol,
+                                         b.mu, b.lam, b.wr, b.W, b.gdt, b.gvol, b.grtol),
+                "adjoint: stage update");
+        HIPCALL(c, tdt::launch_res_apply(c, b.Jst[k], b.Jpst[k], b.wr, a.tptr, a.tlist, a.part,
+                                         a.g, a.gp), "adjoint: residual");
+        if (k > 0)
+            HIPCALL(c, hipMemcpyAsync(b.mu, a.g, n5 * sizeof(double), hipMemcpyDeviceToDevice,
+                                      c->stream), "adjoint: copy");
+        else
+            HIPCALL(c, tdt::launch_add(c, (int)n5, a.g, b.lam), "adjoint: accumulate");
+    }
+    HIPCALL(c, tdt::launch_op_vjp(c, b.JE, b.JEp, nullptr, b.W, b.gdt, b.gvol, b.grtol, b.tptr,
+                                  b.tlist, a.nptr, a.nlist, b.part, b.g, b.gpx, nullptr),
+            "adjoint: operators");
+    HIPCALL(c, tdt::launch_add(c, (int)n5, b.g, b.lam), "adjoint: accumulate");
+    return TDT_OK;
+}
+
 int download_params(tdt_ctx *c, double gp[3]) {
     HIPCALL(c, hipMemcpyAsync(gp, c->iadj->gp, 3 * sizeof(double), hipMemcpyDeviceToHost,
                               c->stream), "download");
@@ -390,6 +452,90 @@ int tdt_iteration_vjp(tdt_ctx *c, const double *cst, double rinit, const double 
         gparam[2] = gp[2] * c->params_rtred;   // RTOL = RINIT * RTRED
     }
     return TDT_OK;
+}
+
+int tdt_adjoint_fixed_point(tdt_ctx *c, const double *cst, double rinit, const double *dj_dc,
+                            int32_t max_iter, double tol, double *lam, double *gparam,
+                            double *gxy, int32_t *iters, double *change) {
+    CHECK_SINGLE(c);
+    if (!cst || !dj_dc || !lam || max_iter < 1 || !(tol >= 0.0)) return TDT_ERR_ARG;
+    int rc = iadj_setup(c);
+    if (!rc && gxy) rc = iadj_xy(c);
+    if (rc) return rc;
+    tdt::IterAdjBufs &b = *c->iadj;
+    const int n = c->pm.n;
+    const size_t n5 = (size_t)5 * n;
+    std::vector<double> keep(n5);
+    HIPCALL(c, tdt::launch_pack(c, c->D.c, b.g), "adjoint: save state");
+    HIPCALL(c, hipMemcpyAsync(keep.data(), b.g, n5 * sizeof(double), hipMemcpyDeviceToHost,
+                              c->stream), "adjoint: save state");
+    const double rinit0 = c->rinit;
+    set_rinit(c, rinit);
+    rc = upload_cells(c, cst, b.qn);
+    if (!rc) rc = upload_cells(c, dj_dc, b.mu);
+    if (!rc) rc = fixed_point_prepare(c, b.qn);
+    if (!rc) HIPCALL(c, hipMemcpyAsync(b.gJ, b.mu, n5 * sizeof(double), hipMemcpyDeviceToDevice,
+                                       c->stream), "adjoint: copy");
+    // lam_{k+1} = Phi_Q^T lam_k + dJ/dQ from lam_0 = dJ/dQ; the change is checked
+    // on the host every 25 iterations
+    std::vector<double> h0(n5), h1(n5);
+    int it = 0;
+    double rel = INFINITY;
+    while (!rc && it < max_iter && !(rel <= tol)) {
+        const bool check = (it + 1) % 25 == 0 || it + 1 == max_iter;
+        if (check)   // lam_k (frozen_vjp reuses b.mu for the stages' adjoints)
+            HIPCALL(c, hipMemcpyAsync(h0.data(), b.mu, n5 * sizeof(double),
+                                      hipMemcpyDeviceToHost, c->stream), "download");
+        rc = frozen_vjp(c, b.qn);
+        if (rc) break;
+        HIPCALL(c, tdt::launch_add(c, (int)n5, b.gJ, b.lam), "adjoint: accumulate");
+        ++it;
+        if (check) {
+            HIPCALL(c, hipMemcpyAsync(h1.data(), b.lam, n5 * sizeof(double),
+                                      hipMemcpyDeviceToHost, c->stream), "download");
+            rc = sync(c);
+            if (rc) break;
+            double dmax = 0.0, lmax = 0.0;
+            for (size_t k = 0; k < n5; ++k) {
+                dmax = std::max(dmax, std::fabs(h1[k] - h0[k]));
+                lmax = std::max(lmax, std::fabs(h1[k]));
+            }
+            rel = lmax > 0.0 ? dmax / lmax : 0.0;
+            if (!std::isfinite(rel)) {
+                c->err = "fixed-point adjoint: the iteration diverged";
+                rc = TDT_ERR_NONFINITE;
+                break;
+            }
+        }
+        HIPCALL(c, hipMemcpyAsync(b.mu, b.lam, n5 * sizeof(double), hipMemcpyDeviceToDevice,
+                                  c->stream), "adjoint: copy");
+    }
+    // lam* out, then one full reverse iteration with mu = lam* for Phi_p^T lam*
+    // and Phi_xy^T lam*
+    if (!rc) rc = download_cells(c, b.mu, lam);
+    if (!rc) HIPCALL(c, zero(c, b.gp, 3), "adjoint: clear");
+    if (!rc && gxy) HIPCALL(c, zero(c, b.gxy, (size_t)2 * c->pm.m), "adjoint: clear");
+    if (!rc) rc = iteration_vjp_core(c, b.qn, gxy != nullptr);
+    double gp[3] = {0.0, 0.0, 0.0};
+    if (!rc) rc = download_params(c, gp);
+    if (!rc && gxy) rc = download_nodes(c, gxy);
+    set_rinit(c, rinit0);
+    int rc2 = TDT_OK;
+    if (hipMemcpyAsync(b.qn, keep.data(), n5 * sizeof(double), hipMemcpyHostToDevice,
+                       c->stream) != hipSuccess ||
+        tdt::launch_unpack(c, b.qn) != hipSuccess)
+        rc2 = TDT_ERR_DEVICE;
+    if (!rc2) rc2 = sync(c);
+    if (iters) *iters = it;
+    if (change) *change = rel;
+    if (rc) return rc;
+    if (rc2) return rc2;
+    if (gparam) {
+        gparam[0] = gp[0];
+        gparam[1] = gp[1];
+        gparam[2] = gp[2] * c->params_rtred;   // RTOL = RINIT * RTRED
+    }
+    return TDT_OK;   // *change > tol: max_iter reached first (the caller decides)
 }
 
 int tdt_adjoint_forward(tdt_ctx *c, int32_t niter, const double *c0, double rinit,

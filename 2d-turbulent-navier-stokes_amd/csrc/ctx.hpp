@@ -103,6 +103,10 @@ struct IterAdjBufs {
     double *part = nullptr, *gp = nullptr, *gxy = nullptr;   // gp: P0, T0, RTOL
     // unrolled run: checkpoints every `stride` iterations, one segment of states,
     // the final state
+    // fixed-point adjoint: the residual blocks at the four stage states of the
+    // fixed state, kept for the whole iteration
+    double *Jst[4] = {}, *Jpst[4] = {};
+    double *gJ = nullptr, *gpx = nullptr;    // dJ/dQ at the fixed state; parameter scratch
     std::vector<double *> ck;
     std::vector<double *> seg;
     double *qlast = nullptr;
@@ -158,6 +162,11 @@ hipError_t launch_postprocess(tdt_ctx *c, double *out);
 
 // Residual and its adjoint (adjoint.hip); qs, v, r, g: [5][n] internal order.
 hipError_t launch_res_eval(tdt_ctx *c, const double *qs, double *r);
+// ... in two steps: the per-cell blocks at a state, then their transpose applied to v
+hipError_t launch_res_jac(tdt_ctx *c, const double *qs, double *J, double *Jp, double *Jx);
+hipError_t launch_res_apply(tdt_ctx *c, const double *J, const double *Jp, const double *v,
+                            const int *tptr, const int2 *tlist, double *part, double *g,
+                            double *gp);
 hipError_t launch_res_vjp(tdt_ctx *c, const double *qs, const double *v, const int *tptr,
                           const int2 *tlist, double *J, double *Jp, double *part, double *g,
                           double *gp, double *Jx, const int *nptr, const int2 *nlist,

@@ -102,6 +102,8 @@ def load(path: str = LIB_PATH):
     L.tdt_operators_eval.argtypes = [vp, _dp, _dp]
     L.tdt_operators_vjp.argtypes = [vp, _dp, ctypes.c_double, _dp, _dp, _dp, _dp]
     L.tdt_iteration_vjp.argtypes = [vp, _dp, ctypes.c_double, _dp, _dp, _dp, _dp]
+    L.tdt_adjoint_fixed_point.argtypes = [vp, _dp, ctypes.c_double, _dp, ctypes.c_int32,
+                                          ctypes.c_double, _dp, _dp, _dp, _ip, _dp]
     L.tdt_adjoint_forward.argtypes = [vp, ctypes.c_int32, _dp, ctypes.c_double, _dp]
     L.tdt_adjoint_reverse.argtypes = [vp, _dp, _dp, _dp, _dp]
     _lib = L
@@ -294,6 +296,24 @@ class Solver:
         self._check(self.L.tdt_iteration_vjp(self.h, _d(c), float(rinit), _d(mu), _d(lam),
                                              _d(gp), _d(gxy) if nodes else None))
         return (lam, gp, gxy) if nodes else (lam, gp)
+
+    def adjoint_fixed_point(self, c: np.ndarray, rinit: float, dj_dc: np.ndarray,
+                            max_iter: int = 20000, tol: float = 1e-12, nodes: bool = False):
+        """Adjoint at a steady state c: (lam = (I - Phi_Q^T)^-1 dJ/dc [5, neles],
+        lam^T dPhi/d(P0, T0, RINIT) [3], [lam^T dPhi/d(X, Y) [2, nods] with nodes],
+        iterations done, last relative change)."""
+        c = np.ascontiguousarray(c, dtype=np.float64).reshape(5, self.neles)
+        g = np.ascontiguousarray(dj_dc, dtype=np.float64).reshape(5, self.neles)
+        lam = np.empty((5, self.neles))
+        gp = np.zeros(3)
+        gxy = np.empty((2, self.nods)) if nodes else None
+        it = ctypes.c_int32(0)
+        ch = ctypes.c_double(0.0)
+        self._check(self.L.tdt_adjoint_fixed_point(self.h, _d(c), float(rinit), _d(g),
+                                                   int(max_iter), float(tol), _d(lam), _d(gp),
+                                                   _d(gxy) if nodes else None,
+                                                   ctypes.byref(it), ctypes.byref(ch)))
+        return (lam, gp, gxy, it.value, ch.value) if nodes else (lam, gp, it.value, ch.value)
 
     def adjoint_forward(self, niter: int, c0=None, rinit: float = 0.0) -> np.ndarray:
         """niter iterations from c0 (None: the cold start), keeping states for

@@ -457,6 +457,9 @@ def adjoint_study(niter=200, ref_iters=40, fd=True, reference=True):
                   flush=True)
         if reference:
             jr, gr, _, _ = gradients(s, ref_iters)
+    out["fixed_point"] = fixed_point_check()
+    print(f"bench: fixed-point adjoint {out['fixed_point']['iterations']} iterations, rel err "
+          f"{out['fixed_point']['rel_err']}", file=sys.stderr, flush=True)
     if reference:
         # central differences of the oracle's J over ref_iters iterations
         sys.path.insert(0, os.path.join(REPO, "oracle"))
@@ -485,6 +488,43 @@ def adjoint_study(niter=200, ref_iters=40, fd=True, reference=True):
         print(f"bench: adjoint vs oracle differences {out['reference_check']['rel_err']}",
               file=sys.stderr, flush=True)
     return out
+
+
+def fixed_point_check():
+    """The adjoint iterated to its fixed point at a steady state: the 16x8 channel
+    whose uniform cold start is steady (total-condition inlet, outlet at the
+    flow's static pressure, symmetry walls); J = outlet mass flow.  dJ/dP0 and
+    dJ/dT0 from lam^T dPhi/dp against the closed-form derivative of the steady
+    state's mass flow (the outlet pressure pins the Mach number)."""
+    import dataclasses
+    g, p0, t0, mach = 1.4, 1.2e5, 298.0, 0.3
+    mesh = gridgen.channel(nx=16, ny=8, length=100.0, height=20.0, h1=2.0, jitter=0.02,
+                           bc={"left": 2, "right": 3, "bottom": 4, "top": 4})
+    pamb = p0 / (1.0 + 0.5 * (g - 1.0) * mach * mach) ** (g / (g - 1.0))
+    flow = gridgen.flow_channel(p0=p0, t0=t0, amachinit=mach, pamb=pamb)
+    out = objective.outlet_faces(mesh)
+    sx = mesh.y_mm[out[:, 2]] * 1e-3 - mesh.y_mm[out[:, 1]] * 1e-3
+
+    def mdot(pp, tt):
+        rr = 8314.0 / flow.amol
+        m2 = 2.0 / (g - 1.0) * ((pp / pamb) ** ((g - 1.0) / g) - 1.0)
+        t = tt / (1.0 + 0.5 * (g - 1.0) * m2)
+        return float(pamb / (rr * t) * np.sqrt(m2 * g * rr * t) * np.sum(sx))
+    with tdt.Solver(mesh, dataclasses.replace(flow)) as s:
+        rinit = s.cold_start()
+        c = s.get_state()
+        dj = np.zeros_like(c)
+        np.add.at(dj[1], out[:, 0], sx)
+        t0_ = time.perf_counter()
+        _, gp, iters, change = s.adjoint_fixed_point(c, rinit, dj, max_iter=60000, tol=1e-12)
+        el = time.perf_counter() - t0_
+    exact = {"p0": (mdot(p0 * (1 + 1e-6), t0) - mdot(p0 * (1 - 1e-6), t0)) / (2e-6 * p0),
+             "t0": (mdot(p0, t0 * (1 + 1e-6)) - mdot(p0, t0 * (1 - 1e-6))) / (2e-6 * t0)}
+    return {"case": "16x8 channel, uniform steady state, J = outlet mass flow", "iterations": iters,
+            "relative_change": float(f"{change:.3e}"), "seconds": round(el, 2),
+            "dJ": {"p0": gp[0], "t0": gp[1]}, "closed_form": exact,
+            "rel_err": {k: float(f"{abs(gp[i] - exact[k]) / abs(exact[k]):.3e}")
+                        for i, k in enumerate(("p0", "t0"))}}
 
 
 _SWEEP = []   # the CPU sweep's cases, generated before its workers fork

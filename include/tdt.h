@@ -173,6 +173,19 @@ int tdt_operators_vjp(tdt_ctx *ctx, const double *c, double rinit, const double 
  * is set to it); its routine-level arrays are overwritten. */
 int tdt_iteration_vjp(tdt_ctx *ctx, const double *c, double rinit, const double *mu,
                       double *lam, double *gparam, double *gxy);
+/* Fixed-point adjoint at a steady state c (with RINIT = rinit; c = Phi(c), Phi one
+ * iteration): lam = (I - Phi_Q^T)^-1 dj_dc by the iteration lam <- Phi_Q^T lam +
+ * dj_dc from lam = dj_dc, the RK stages recomputed once at c and their blocks
+ * kept, until the largest change relative to the largest |lam| is <= tol (checked
+ * every 25 iterations) or max_iter; then gparam ([3] or NULL) = lam^T
+ * dPhi/d(P0, T0, RINIT) and gxy ([2][nods] or NULL) = lam^T dPhi/d(X, Y) per
+ * grid.in millimetre, so dJ/dp = dJ/dp|explicit + gparam for the steady state's
+ * J.  iters, change (may be NULL): iterations done, last relative change; a
+ * change above tol means max_iter came first.  TDT_ERR_NONFINITE if the
+ * iteration diverges.  The context's C is kept (COLD is set to it). */
+int tdt_adjoint_fixed_point(tdt_ctx *ctx, const double *c, double rinit, const double *dj_dc,
+                            int32_t max_iter, double tol, double *lam, double *gparam,
+                            double *gxy, int32_t *iters, double *change);
 /* niter iterations from c0 with RINIT = rinit (c0 NULL: from the cold start of
  * turbnstwod.f90:53-71, rinit ignored), as tdt_iterate computes them; cn = Q^N.
  * States are kept every ~sqrt(niter) iterations for tdt_adjoint_reverse.  The

@@ -504,3 +504,66 @@ def test_duct_design_gradients():
                 j.append(run_j(s, dataclasses.replace(mesh, x_mm=x, y_mm=y), flow))
             s.set_nodes(mesh.x_mm, mesh.y_mm)
             check((j[0] - j[1]) / (2 * h), ad, abs(ad), 1e-5, f"bump{k + 1}")
+
+
+def uniform_channel(p0=1.2e5, t0=298.0, mach=0.3):
+    """A channel whose cold start is a steady state of the iteration: total
+    conditions at the inlet (type 2), the uniform flow's static pressure at the
+    outlet (type 3 with PAMB), symmetry above and below (type 4)."""
+    from tdt import gridgen
+    g = 1.4
+    mesh = gridgen.channel(nx=16, ny=8, length=100.0, height=20.0, h1=2.0, jitter=0.02,
+                           bc={"left": 2, "right": 3, "bottom": 4, "top": 4})
+    pamb = p0 / (1.0 + 0.5 * (g - 1.0) * mach * mach) ** (g / (g - 1.0))
+    return mesh, gridgen.flow_channel(p0=p0, t0=t0, amachinit=mach, pamb=pamb)
+
+
+def steady_outlet_mass_flow(mesh, flow, p0, t0):
+    """Outlet mass flow of the steady state for (P0, T0) at the flow's PAMB:
+    uniform flow at the Mach number whose static pressure from P0 is PAMB."""
+    from tdt import objective
+    g, rr = flow.gama, 8314.0 / flow.amol
+    m2 = 2.0 / (g - 1.0) * ((p0 / flow.pamb) ** ((g - 1.0) / g) - 1.0)
+    t = t0 / (1.0 + 0.5 * (g - 1.0) * m2)
+    rho = flow.pamb / (rr * t)
+    u = np.sqrt(m2 * g * rr * t)
+    out = objective.outlet_faces(mesh)
+    sx, _, _ = objective._scx(mesh, 0, out[:, 1], out[:, 2])
+    return float(rho * u * np.sum(sx))
+
+
+def test_fixed_point_adjoint_at_a_steady_state():
+    """The adjoint iterated to its fixed point, lam = Phi_Q^T lam + dJ/dQ, at a
+    steady state of the iteration (the uniform channel): J = outlet mass flow;
+    dJ/dP0 = lam^T dPhi/dP0 must equal the derivative of the steady state's mass
+    flow in closed form (the outlet pressure pins the Mach number), dJ/dT0 too,
+    and RINIT (which only moves inactive floors) contributes nothing."""
+    import tdt
+    from tdt import objective
+    mesh, flow = uniform_channel()
+    with tdt.Solver(mesh, flow) as s:
+        rinit = s.cold_start()
+        c = s.get_state()
+        s.iterate(50, history=False)
+        scale = np.abs(c).max(axis=1)
+        scale[2] = scale[1]                          # rho v is 0: measure it against rho u
+        drift = np.max(np.abs(s.get_state() - c) / scale[:, None])
+        assert drift < 1e-12, drift                  # a fixed point of the iteration
+        out = objective.outlet_faces(mesh)
+        sx, _, _ = objective._scx(mesh, 0, out[:, 1], out[:, 2])
+        dj = np.zeros_like(c)
+        np.add.at(dj[1], out[:, 0], sx)              # J = sum (rho u) SC_x at the outlet
+        assert abs(float(np.sum(dj * c)) - steady_outlet_mass_flow(mesh, flow, flow.p0,
+                                                                   flow.t0)) < 1e-9
+        lam, gp, iters, change = s.adjoint_fixed_point(c, rinit, dj, max_iter=60000, tol=1e-12)
+        assert change <= 1e-12, (iters, change)
+        for k, name in ((0, "p0"), (1, "t0")):
+            base = getattr(flow, name)
+            h = 1e-6 * base
+            args = [dict(p0=flow.p0, t0=flow.t0) for _ in range(2)]
+            args[0][name] = base + h
+            args[1][name] = base - h
+            fd = (steady_outlet_mass_flow(mesh, flow, **args[0])
+                  - steady_outlet_mass_flow(mesh, flow, **args[1])) / (2 * h)
+            check(fd, gp[k], abs(fd), 1e-6, name)
+        assert abs(gp[2]) * rinit < 1e-12 * abs(gp[0]) * flow.p0
