@@ -67,28 +67,51 @@ __device__ __forceinline__ void block_atomic_min_pos(double x, double *slot) {
     }
 }
 
+// Combination of partial record field q: a sum for the sums of squares, a
+// maximum (keeping a, as wave_max does, unless b is larger) for the rest.
+__device__ __forceinline__ double part_op(int q, double a, double b) {
+    return q < 5 ? a + b : (b > a ? b : a);
+}
+
 // Writes this block's partial sums of squares, maxima and non-finite flag.
+// The eleven row butterflies run side by side (no value waits on another's
+// DPP chain); each row's result goes through LDS, and one thread per field
+// combines rows 0..3 of wave 0, 1, ... in that order: the same operations, in
+// the same order, as wave_sum / wave_max followed by a fold over the waves.
 __device__ __forceinline__ void block_partials_write(const double *sq, const double *mx,
                                                      int bad, double *out) {
-    __shared__ double sh[16][kPartStride];
-    int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
-    int nw = (blockDim.x + kWave - 1) / kWave;
+    __shared__ double sh[16 * 4][kPartStride];   // [wave * 4 + row of 16 lanes][field]
+    const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+    const int nw = (blockDim.x + kWave - 1) / kWave;
+    double x[kPartStride];
 #pragma unroll
     for (int m = 0; m < 5; ++m) {
-        double s = wave_sum(sq[m]);
-        double x = wave_max(mx[m]);
-        if (lane == 0) { sh[w][m] = s; sh[w][5 + m] = x; }
+        x[m] = sq[m];
+        x[5 + m] = mx[m];
     }
-    double b = wave_max((double)bad);
-    if (lane == 0) sh[w][10] = b;
+    x[10] = (double)bad;
+#pragma unroll
+    for (int q = 0; q < kPartStride; ++q) x[q] = part_op(q, x[q], row_ror<8>(x[q]));
+#pragma unroll
+    for (int q = 0; q < kPartStride; ++q) x[q] = part_op(q, x[q], row_ror<4>(x[q]));
+#pragma unroll
+    for (int q = 0; q < kPartStride; ++q) x[q] = part_op(q, x[q], row_ror<2>(x[q]));
+#pragma unroll
+    for (int q = 0; q < kPartStride; ++q) x[q] = part_op(q, x[q], row_ror<1>(x[q]));
+    if ((lane & 15) == 0) {
+#pragma unroll
+        for (int q = 0; q < kPartStride; ++q) sh[w * 4 + (lane >> 4)][q] = x[q];
+    }
     __syncthreads();
     if (threadIdx.x < kPartStride) {
-        int q = threadIdx.x;
-        double acc = sh[0][q];
-        for (int k = 1; k < nw; ++k) {
-            double y = sh[k][q];
-            if (q < 5) acc = acc + y;
-            else acc = y > acc ? y : acc;
+        const int q = threadIdx.x;
+        double acc = 0.0;
+        for (int k = 0; k < nw; ++k) {
+            double r = sh[4 * k][q];
+            r = part_op(q, r, sh[4 * k + 1][q]);
+            r = part_op(q, r, sh[4 * k + 2][q]);
+            r = part_op(q, r, sh[4 * k + 3][q]);
+            acc = k == 0 ? r : part_op(q, acc, r);
         }
         out[q] = acc;
     }
