@@ -121,7 +121,7 @@ __device__ __forceinline__ void block_partials_write(const double *sq, const dou
 // out[0..4] = sqrt(sum / n), out[5..9] = max, *flag |= any non-finite.
 __device__ __forceinline__ void reduce_partials(const double *part, int nblk, int n,
                                                 double *out, int *flag) {
-    __shared__ double sh[16][kPartStride];
+    __shared__ double sh[16 * 4][kPartStride];   // [wave * 4 + row of 16 lanes][field]
     double acc[kPartStride];
     for (int q = 0; q < kPartStride; ++q) acc[q] = 0.0;
     for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
@@ -129,20 +129,32 @@ __device__ __forceinline__ void reduce_partials(const double *part, int nblk, in
         for (int q = 0; q < 5; ++q) acc[q] = acc[q] + p[q];
         for (int q = 5; q < kPartStride; ++q) acc[q] = p[q] > acc[q] ? p[q] : acc[q];
     }
-    int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
-    int nw = (blockDim.x + kWave - 1) / kWave;
-    for (int q = 0; q < kPartStride; ++q) {
-        double v = q < 5 ? wave_sum(acc[q]) : wave_max(acc[q]);
-        if (lane == 0) sh[w][q] = v;
+    // as in block_partials_write: row butterflies side by side, rows 0..3 of
+    // each wave combined in order through LDS
+    const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+    const int nw = (blockDim.x + kWave - 1) / kWave;
+#pragma unroll
+    for (int q = 0; q < kPartStride; ++q) acc[q] = part_op(q, acc[q], row_ror<8>(acc[q]));
+#pragma unroll
+    for (int q = 0; q < kPartStride; ++q) acc[q] = part_op(q, acc[q], row_ror<4>(acc[q]));
+#pragma unroll
+    for (int q = 0; q < kPartStride; ++q) acc[q] = part_op(q, acc[q], row_ror<2>(acc[q]));
+#pragma unroll
+    for (int q = 0; q < kPartStride; ++q) acc[q] = part_op(q, acc[q], row_ror<1>(acc[q]));
+    if ((lane & 15) == 0) {
+#pragma unroll
+        for (int q = 0; q < kPartStride; ++q) sh[w * 4 + (lane >> 4)][q] = acc[q];
     }
     __syncthreads();
     if (threadIdx.x < kPartStride) {
-        int q = threadIdx.x;
-        double a = sh[0][q];
-        for (int k = 1; k < nw; ++k) {
-            double y = sh[k][q];
-            if (q < 5) a = a + y;
-            else a = y > a ? y : a;
+        const int q = threadIdx.x;
+        double a = 0.0;
+        for (int k = 0; k < nw; ++k) {
+            double r = sh[4 * k][q];
+            r = part_op(q, r, sh[4 * k + 1][q]);
+            r = part_op(q, r, sh[4 * k + 2][q]);
+            r = part_op(q, r, sh[4 * k + 3][q]);
+            a = k == 0 ? r : part_op(q, a, r);
         }
         if (q < 5) out[q] = sqrt(a / (double)n);
         else if (q < 10) out[q] = a;
