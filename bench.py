@@ -650,6 +650,10 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="run a plain device copy of MIB MiB (HBM-counter calibration) and exit")
     args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); multi-GPU "
+              "sharding is a design, not an implementation", file=sys.stderr)
+        sys.exit(2)
     if args.copy_probe:
         copy_probe(args.copy_probe)
         return
@@ -682,10 +686,6 @@ def main():
         print(f"bench: {args.precision} {mesh.neles * args.steps / el / 1e6:.1f} Mcell-iters/s "
               f"kernel set {ks} kernels(ms) {np.round(kt, 5).tolist()}", flush=True)
         return
-    if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); multi-GPU "
-              "sharding is a design, not an implementation", file=sys.stderr)
-        sys.exit(2)
     mk_mesh, mk_flow = gridgen.WORKLOADS[args.workload]
     mesh, flow = mk_mesh(), mk_flow()
     n = mesh.neles
