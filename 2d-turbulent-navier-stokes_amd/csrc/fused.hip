@@ -80,6 +80,14 @@ T *dalloc(FusedBufs &B, size_t count) {
     return static_cast<T *>(p);
 }
 
+// Frees one buffer dalloc made (the caller has drained the stream).
+void drelease(FusedBufs &B, void *p) {
+    auto it = std::find(B.own.begin(), B.own.end(), p);
+    if (it == B.own.end()) return;
+    (void)hipFree(p);
+    B.own.erase(it);
+}
+
 }  // namespace
 
 int fused_setup(tdt_ctx *c) {
@@ -402,7 +410,14 @@ int fused_iterate(tdt_ctx *c, int niter, double *l2, double *linf, int *first_no
             c->err = "pinned allocation of the monitor history failed";
             return TDT_ERR_DEVICE;
         }
+        // the old buffers: the stream has no work left that reads them
+        hipError_t es = hipStreamSynchronize(c->stream);
+        if (es != hipSuccess) {
+            c->err = std::string("monitor history: ") + hipGetErrorString(es);
+            return TDT_ERR_DEVICE;
+        }
         if (B.hist_host) (void)hipHostFree(B.hist_host);
+        if (F.hist) drelease(B, F.hist);
         B.hist_host = hh;
         F.hist = h;
         B.hist_cap = cap;
@@ -447,9 +462,10 @@ int fused_iterate(tdt_ctx *c, int niter, double *l2, double *linf, int *first_no
         for (int q = 0; q < kNumFusedKernels; ++q) c->ktimes[q] /= (niter > 0 ? niter : 1);
         if ((B.kset & kSt1Mask) != kSt1Split) c->ktimes[1] = 0.0;   // no k_dv launch
     } else if (e == hipSuccess) {
+        // both graphs are captured on the first call, whatever its length, so
+        // that later calls only replay them
         if (!B.exec1) e = build_graph(c, B, 1, &B.graph1, &B.exec1);
-        if (e == hipSuccess && !B.execK && niter >= B.chunk)
-            e = build_graph(c, B, B.chunk, &B.graphK, &B.execK);
+        if (e == hipSuccess && !B.execK) e = build_graph(c, B, B.chunk, &B.graphK, &B.execK);
         int done = 0;
         while (e == hipSuccess && niter - done >= B.chunk && B.execK) {
             e = hipGraphLaunch(B.execK, c->stream);

@@ -3,13 +3,16 @@
 
 A step is one full Jameson RK4 iteration (time step, JST dissipation, stage-1
 viscous fluxes, source terms, ghost cells, four stage updates, monitor) over
-the whole grid.  The workload is the 385 x 129 axisymmetric
-converging-diverging duct (49,665 cells), synthetic grid from the seed.
+the whole grid.  The headline workload is the 2049 x 513 supersonic 15-degree
+compression ramp (1,051,137 cells; synthetic grid from the seed, cold start),
+the largest single-GPU configuration and the one whose working set does not
+fit in the 256 MB Infinity Cache, in FP64 (the parity path).
 
-Prints ONE JSON line: Mcell-iters/s of the FP64 path (the parity-checked
-path), the FP32-mixed path beside it with its error against FP64, the
-roofline of the dominant kernel and the reference CPU path timed on this
-host.
+Prints ONE JSON line: Mcell-iters/s of the FP64 path, the roofline of its
+dominant kernel and of the whole iteration (two byte conventions), the
+reference executable timed on this host, and beside them the FP32-mixed path
+(with its error at run length), the 385 x 129 duct, the 97 x 49 channel, the
+64-case batched sweep and the discrete adjoint study.
 """
 from __future__ import annotations
 
@@ -72,6 +75,13 @@ def alg_bytes_per_cell(precision: str, m_over_n: float, local: int, kset: str = 
             "k_stage3": st23, "k_stage4": st4}
 
 
+# Bytes per cell-iteration by the convention the byte budget was first set in:
+# every per-cell array counted once per pass (time step, dissipation, node
+# averages, four stages) that reads or writes it, at its storage width; FP64
+# 860 + 81 * 8, FP32-mixed 860 + 81 * 4.
+SURVEY_BYTES = {"fp64": 1508.0, "fp32-mixed": 1184.0}
+
+
 def load_traffic(workload: str, precision: str, kset: str):
     """Measured HBM bytes per launch of each kernel (rocprofv3 FETCH_SIZE + WRITE_SIZE,
     scaled by a copy kernel of known size), if a committed summary exists for the
@@ -96,6 +106,7 @@ def roofline(workload, precision, mesh, flow, kt, kset, elapsed, steps):
     name = KERNELS[dom]
     achieved = b[name] * n / (kt[dom] * 1e-3) / 1e9
     traffic = load_traffic(workload, precision, kset)
+    rate = n * steps / elapsed                       # cell-iterations per second
     return {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": (traffic or {}).get(name),
@@ -103,9 +114,12 @@ def roofline(workload, precision, mesh, flow, kt, kset, elapsed, steps):
             "avg_launch_ms": round(float(kt[dom]), 5),
             "kernel_set": kset,
             "alg_bytes_per_cell_iter": round(sum(b.values()), 1),
-            "iteration_alg_GBs": round(sum(b.values()) * n * steps / elapsed / 1e9, 1),
-            "iteration_frac": round(sum(b.values()) * n * steps / elapsed / 1e9 / HBM_PEAK_GBS,
-                                    4)}
+            "iteration_alg_GBs": round(sum(b.values()) * rate / 1e9, 1),
+            "iteration_frac": round(sum(b.values()) * rate / 1e9 / HBM_PEAK_GBS, 4),
+            "survey_bytes_per_cell_iter": SURVEY_BYTES[precision],
+            "iteration_alg_GBs_survey": round(SURVEY_BYTES[precision] * rate / 1e9, 1),
+            "iteration_frac_survey": round(SURVEY_BYTES[precision] * rate / 1e9 / HBM_PEAK_GBS,
+                                           4)}
 
 
 def run_gpu(mesh, flow, precision, steps, warmup, ordering="auto"):
@@ -274,27 +288,6 @@ def other_case(workload, stream_gbs, steps, warmup, converge, cpu):
     if cpu:
         out["cpu_baseline"] = cpu_baseline(mesh, flow)
         print(f"bench: {workload} CPU {out['cpu_baseline']}", file=sys.stderr, flush=True)
-    return out
-
-
-def large_case(stream_gbs, steps=200, warmup=50, converge=20000, cpu=True):
-    """The 2049x513 compression ramp (1.05 M cells: enough wavefronts to fill
-    the GPU), for the HBM-bound regime the 385x129 duct cannot reach.  With its
-    type-3 (extrapolated) Mach 2.5 inflow the case goes non-finite within 4000
-    iterations, so `inflow_type3` records where each precision does, and the
-    FP32-mixed vs FP64 comparison runs the case's 20000 iterations on the same
-    ramp with the type-7 (fixed p, T, u) inflow."""
-    out = other_case("ramp_2049x513", stream_gbs, steps, warmup, 0, cpu)
-    if converge > 0:
-        mk_mesh, mk_flow = gridgen.WORKLOADS["ramp_2049x513"]
-        out["inflow_type3"] = converged_comparison(mk_mesh(), mk_flow(), 4000)
-        print(f"bench: ramp_2049x513 (type-3 inflow) {out['inflow_type3']}", file=sys.stderr,
-              flush=True)
-        mk_mesh, mk_flow = gridgen.WORKLOADS["ramp_2049x513_t7"]
-        out["converged"] = {"workload": f"ramp_2049x513_t7: {DESCRIBE['ramp_2049x513_t7']}",
-                            **converged_comparison(mk_mesh(), mk_flow(), converge)}
-        print(f"bench: ramp_2049x513_t7 converged comparison {out['converged']}",
-              file=sys.stderr, flush=True)
     return out
 
 
@@ -613,7 +606,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--workload", default="duct_385x129", choices=sorted(gridgen.WORKLOADS))
+    ap.add_argument("--workload", default="ramp_2049x513", choices=sorted(gridgen.WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-mixed", action="store_true", help="skip the FP32-mixed run")
     ap.add_argument("--no-stream", action="store_true", help="skip the copy-bandwidth probe")
@@ -636,8 +629,10 @@ def main():
                     help="with --adjoint: skip the finite-difference checks")
     ap.add_argument("--no-sequential", action="store_true",
                     help="with --batch: skip the one-after-another runs")
-    ap.add_argument("--no-large", action="store_true",
-                    help="skip the 2049x513 ramp roofline run beside the main workload")
+    ap.add_argument("--no-duct", action="store_true",
+                    help="skip the 385x129 duct run beside the main workload")
+    ap.add_argument("--no-sweep", action="store_true",
+                    help="skip the 64-case batched sweep beside the main workload")
     ap.add_argument("--no-small", action="store_true",
                     help="skip the 97x49 channel run beside the main workload")
     ap.add_argument("--converge", type=int, default=-1, metavar="ITERS",
@@ -689,6 +684,7 @@ def main():
     mk_mesh, mk_flow = gridgen.WORKLOADS[args.workload]
     mesh, flow = mk_mesh(), mk_flow()
     n = mesh.neles
+    ramp = args.workload == "ramp_2049x513"
     print(f"bench: {args.workload} neles={n} nghosts={mesh.nghosts} steps={args.steps} "
           f"warmup={args.warmup}", file=sys.stderr, flush=True)
 
@@ -698,6 +694,16 @@ def main():
     print(f"bench: fp64 {v64:.1f} Mcell-iters/s ({el64 * 1e3 / args.steps:.4f} ms/iter) "
           f"kernel set {ks64} kernels(ms) {np.round(kt64, 5).tolist()}", file=sys.stderr,
           flush=True)
+    stream = None
+    if not args.no_stream:
+        try:
+            stream = round(stream_copy_gbs(), 1)
+        except Exception as exc:                  # torch missing or no device memory
+            print(f"bench: stream copy not measured: {exc}", file=sys.stderr, flush=True)
+    roof = roofline(args.workload, "fp64", mesh, flow, kt64, ks64, el64, args.steps)
+    roof["stream_GBs"] = stream
+    if stream:
+        roof["frac_of_stream"] = round(roof["achieved"] / stream, 4)
     mixed = None
     if not args.no_mixed:
         el32, st32, _, bad32, kt32, ks32 = run_gpu(mesh, flow, "fp32-mixed", args.steps,
@@ -707,38 +713,47 @@ def main():
         b = tdt.primitives_from_state(st64, flow)
         err = {k: float(np.linalg.norm(a[k] - b[k]) / max(np.linalg.norm(b[k]), 1e-300))
                for k in ("p", "rho", "u", "nut")}
-        mixed = {"value": round(v32, 2), "ms_per_step": round(el32 * 1e3 / args.steps, 5),
+        mixed = {"value": round(v32, 2), "unit": "Mcell-iters/s",
+                 "ms_per_step": round(el32 * 1e3 / args.steps, 5),
                  "speedup_vs_fp64": round(v32 / v64, 3),
                  "rel_l2_vs_fp64_after_steps": {k: float(f"{v:.3e}") for k, v in err.items()},
                  "kernel_set": ks32, "kernel_ms": [round(float(x), 5) for x in kt32],
-                 "nonfinite_iter": int(bad32)}
-        print(f"bench: fp32-mixed {v32:.1f} Mcell-iters/s, rel err vs fp64 {err}", file=sys.stderr,
-              flush=True)
-
-    roof = roofline(args.workload, "fp64", mesh, flow, kt64, ks64, el64, args.steps)
-    if mixed is not None:
-        mixed["roofline"] = roofline(args.workload, "fp32-mixed", mesh, flow, kt32, ks32, el32,
-                                     args.steps)
+                 "nonfinite_iter": int(bad32),
+                 "roofline": roofline(args.workload, "fp32-mixed", mesh, flow, kt32, ks32, el32,
+                                      args.steps)}
+        print(f"bench: fp32-mixed {v32:.1f} Mcell-iters/s, kernel set {ks32} kernels(ms) "
+              f"{np.round(kt32, 5).tolist()}, rel err vs fp64 {err}", file=sys.stderr, flush=True)
         citers = flow.niter if args.converge < 0 else args.converge
-        if citers > 0:
+        if citers > 0 and ramp:
+            # the specified type-3 inflow goes non-finite within 4000 iterations (in
+            # the reference too): where each precision does; the error at run length
+            # on the same ramp with the type-7 inflow
+            mixed["inflow_type3"] = converged_comparison(mesh, flow, 4000)
+            print(f"bench: ramp type-3 inflow {mixed['inflow_type3']}", file=sys.stderr,
+                  flush=True)
+            m7, f7 = gridgen.WORKLOADS["ramp_2049x513_t7"]
+            mixed["converged"] = {"workload": f"ramp_2049x513_t7: {DESCRIBE['ramp_2049x513_t7']}",
+                                  **converged_comparison(m7(), f7(), citers)}
+        elif citers > 0:
             mixed["converged"] = converged_comparison(mesh, flow, citers)
+        if citers > 0:
             print(f"bench: converged comparison {mixed['converged']}", file=sys.stderr,
                   flush=True)
 
-    stream = None
-    if not args.no_stream:
-        try:
-            stream = round(stream_copy_gbs(), 1)
-        except Exception as exc:                  # torch missing or no device memory
-            print(f"bench: stream copy not measured: {exc}", file=sys.stderr, flush=True)
-    roof["stream_GBs"] = stream
-    if stream:
-        roof["frac_of_stream"] = round(roof["achieved"] / stream, 4)
-    cpu = None if args.no_cpu else cpu_baseline(mesh, flow)
-    large = None if args.no_large or args.workload.startswith("ramp_2049x513") else \
-        large_case(stream, converge=0 if args.converge == 0 else 20000, cpu=not args.no_cpu)
+    cpu = None if args.no_cpu else cpu_baseline(mesh, flow, target_s=25.0)
+    if cpu:
+        print(f"bench: CPU {cpu}", file=sys.stderr, flush=True)
+    duct = None if args.no_duct or args.workload == "duct_385x129" else \
+        other_case("duct_385x129", stream, 2000, 200, 0 if args.converge == 0 else 20000,
+                   not args.no_cpu)
     small = None if args.no_small or args.workload == "channel_97x49" else \
         small_case(stream, converge=0 if args.converge == 0 else 2000, cpu=not args.no_cpu)
+    sweep = None
+    if not args.no_sweep:
+        try:
+            sweep = batch_sweep(64)
+        except tdt.TdtError as exc:
+            sweep = {"error": str(exc)}
     adjoint = None
     if not args.no_adjoint:
         try:
@@ -755,12 +770,13 @@ def main():
                                f"{n} cells + {mesh.nghosts} ghosts, IAXIS={flow.iaxis}, "
                                f"P0={flow.p0}, T0={flow.t0}, M={flow.amachinit}, CFL={flow.cfl}",
                    "precision": "fp64", "kernel_set": ks64, "nonfinite_iter": int(bad64)},
-        "fp32_mixed": mixed,
         "roofline": roof,
-        "large_case": large,
-        "small_case": small,
-        "adjoint": adjoint,
         "cpu_baseline": cpu,
+        "fp32_mixed": mixed,
+        "duct_case": duct,
+        "small_case": small,
+        "batch_sweep": sweep,
+        "adjoint": adjoint,
     }
     print(json.dumps(line), flush=True)
 

@@ -56,20 +56,28 @@ def parse_final(path):
     return np.frombuffer(raw[4:], dtype="<f8").reshape(5, ne).copy()
 
 
-def run_case(name, mesh, flow, nrun, dump, sample_stride=1, crosscheck=True):
+def run_case(name, mesh, flow, nrun, dump, sample_stride=1, crosscheck=True, big=False,
+             keep_linf=True, periodic=False):
+    """big: the NN = 1,060,000 build of the driver and executable; keep_linf: keep the
+    L-infinity history (False keeps the L2 history only); periodic: keep the head
+    and tail lines of the executable's periodic restart file restart<I10>.in
+    (written every 20000 iterations, turbnstwod.f90:170-182)."""
     work = tempfile.mkdtemp(prefix=f"tdt_golden_{name}_")
+    sfx = "_big" if big else ""
     try:
         flow = dataclasses.replace(flow, niter=nrun + (1 if dump else 0), istart=0)
         caseio.write_case(work, flow, mesh)
         with open(os.path.join(work, "driver.in"), "w") as fh:
             fh.write(f"{nrun} {1 if dump else 0}\n{flow.pinf!r} {flow.aminf!r} {flow.deltabl!r}\n"
                      f"{flow.tflame!r} {flow.ancomb!r} {flow.acomb!r} {flow.rhoprop!r}\n")
-        subprocess.check_call([os.path.join(REF, "ref_driver")], cwd=work, stdout=subprocess.DEVNULL)
+        subprocess.check_call([os.path.join(REF, "ref_driver" + sfx)], cwd=work,
+                              stdout=subprocess.DEVNULL)
         final = parse_final(os.path.join(work, "final.bin"))
         l2, linf = parse_hist(os.path.join(work, "hist.bin"))
         agree = None
         if crosscheck:
-            subprocess.check_call([os.path.join(REF, "turbnstwod")], cwd=work, stdout=subprocess.DEVNULL)
+            subprocess.check_call([os.path.join(REF, "turbnstwod" + sfx)], cwd=work,
+                                  stdout=subprocess.DEVNULL)
             c_ref, *_ = caseio.read_restart(os.path.join(work, "restart.in"), mesh.neles)
             agree = bool(np.array_equal(c_ref, final))
             if not agree:
@@ -78,7 +86,17 @@ def run_case(name, mesh, flow, nrun, dump, sample_stride=1, crosscheck=True):
         data["flow_json"] = np.array(json.dumps(dataclasses.asdict(flow)))
         data["nrun"] = np.array(nrun)
         data["hist_l2"] = l2
-        data["hist_linf"] = linf
+        if keep_linf:
+            data["hist_linf"] = linf
+        if periodic:
+            fn = os.path.join(work, "restart%10d.in" % 20000)
+            lines = open(fn).read().splitlines()
+            c_per, *tail = caseio.read_restart(fn, mesh.neles)
+            if not np.array_equal(c_per, final):
+                raise SystemExit(f"{name}: periodic restart file differs from the final state")
+            data["periodic_name"] = np.array(os.path.basename(fn))
+            data["periodic_head"] = np.array("\n".join(lines[:3]))
+            data["periodic_tail"] = np.array("\n".join(lines[-4:]))
         data["crosschecked"] = np.array(-1 if agree is None else int(agree))
         if sample_stride == 1:
             data["final"] = final
@@ -202,6 +220,21 @@ def main():
     m = gridgen.duct()
     f = gridgen.flow_duct()
     run_case("duct_385x129", m, f, nrun=40, dump=False, sample_stride=7)
+    # ... and its longer runs: 2000 iterations with every cell and both norms, and
+    # the case's NITER = 20000 (L2 history, every cell, the periodic restart file)
+    run_case("duct_385x129_2000", m, f, nrun=2000, dump=False)
+    run_case("duct_385x129_20000", m, f, nrun=20000, dump=False, keep_linf=False,
+             periodic=True)
+    # the 97x49 channel / flat plate as specified: type-7 inlet with its inputs
+    # defined (the executable cannot set them: no cross-check), its NITER = 2000
+    m, f = gridgen.WORKLOADS["channel_97x49"][0](), gridgen.WORKLOADS["channel_97x49"][1]()
+    run_case("channel_97x49", m, f, nrun=2000, dump=False, crosscheck=False)
+    # the 2049x513 ramp with its type-3 (extrapolated) Mach 2.5 inflow: the
+    # reference's own L2 history up to and past the iteration where its state
+    # goes non-finite (about 45 minutes on one core)
+    m, f = gridgen.WORKLOADS["ramp_2049x513"][0](), gridgen.WORKLOADS["ramp_2049x513"][1]()
+    run_case("ramp_2049x513_t3", m, f, nrun=3720, dump=False, sample_stride=4099,
+             crosscheck=False, big=True)
 
 
 if __name__ == "__main__":
