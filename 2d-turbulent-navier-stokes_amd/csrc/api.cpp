@@ -21,6 +21,12 @@ namespace api {
 
 thread_local std::string g_create_error;
 
+}  // namespace api
+
+void api_set_error(const std::string &msg) { api::g_create_error = msg; }
+
+namespace api {
+
 void set_rinit(tdt_ctx *c, double rinit) {
     c->rinit = rinit;
     double rtol = rinit * c->params_rtred;           // RTOLERANCE = RINIT*RTRED

@@ -200,4 +200,8 @@ int fused_kernel_set(tdt_ctx *c);   // kernel set in use, -1 before the first it
 
 inline unsigned nblocks(int count) { return (unsigned)((count + kBlock - 1) / kBlock); }
 
+// Sets the message tdt_last_error(NULL) returns on this thread (failures with no
+// context: tdt_create, tdt_monitor_noalloc).
+void api_set_error(const std::string &msg);
+
 }  // namespace tdt

@@ -106,6 +106,9 @@ def load(path: str = LIB_PATH):
                                           ctypes.c_double, _dp, _dp, _dp, _ip, _dp]
     L.tdt_adjoint_forward.argtypes = [vp, ctypes.c_int32, _dp, ctypes.c_double, _dp]
     L.tdt_adjoint_reverse.argtypes = [vp, _dp, _dp, _dp, _dp]
+    # device pointers and a hipStream_t, passed as addresses
+    L.tdt_monitor_noalloc.argtypes = [vp, ctypes.c_int64, vp, ctypes.c_int32, ctypes.c_int32, vp,
+                                      vp]
     _lib = L
     return L
 

@@ -270,6 +270,26 @@ int tdt_kernel_times(tdt_ctx *ctx, double *times_ms);
 int tdt_kernel_set(tdt_ctx *ctx, int32_t *set);
 int tdt_synchronize(tdt_ctx *ctx);
 
+/* ---- monitor without allocation ---------------------------------------------------
+ * No counterpart in the reference: for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc), the monitor of tdt_monitor (turbnstwod.f90:147-168) on their own
+ * device arrays, with no context.  c: device [5][ldc] = C1..C5 of n cells
+ * (component stride ldc >= n); cold: device [5][n] = COLD1..COLD5; out: device
+ * [11] = the five L2 = sqrt(sum (C-COLD)^2 / n), the five Linf = max |C-COLD|
+ * and a non-finite flag (1.0 if C holds NaN/Inf, else 0.0); save_old != 0 then
+ * sets COLD <- C.  Over the same cells in the same order the values have the
+ * same bits as tdt_monitor's.  Asynchronous on stream (a hipStream_t; NULL =
+ * the null stream): out is valid once the stream has reached it.  Allocates
+ * nothing: the per-block partials live in a static device array of
+ * TDT_MONITOR_NOALLOC_MAX_CELLS / 256 records (5.8 MB), one per GPU, held until
+ * the process ends; n above TDT_MONITOR_NOALLOC_MAX_CELLS gives TDT_ERR_ARG.
+ * All calls on one GPU share that array: calls that overlap on one GPU race.
+ * Errors: tdt_last_error(NULL). */
+#define TDT_MONITOR_NOALLOC_MAX_CELLS (1 << 24)
+int tdt_monitor_noalloc(const double *c, int64_t ldc, double *cold, int32_t n, int32_t save_old,
+                        double *out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
