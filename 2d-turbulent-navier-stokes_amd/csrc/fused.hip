@@ -158,12 +158,16 @@ int fused_setup(tdt_ctx *c) {
     // set; otherwise the first tdt_iterate call picks it by timing (autotune)
     const char *kv = std::getenv("TDT_KERNELS");
     const std::string ks = kv ? kv : "";
-    if (ks == "merged" || ks == "split" || ks == "quad") {
-        B->kset = ks == "merged" ? kSetMerged : ks == "split" ? kSetSplit : kSetQuad;
+    // the face-once viscous set exists for FP32-mixed single cases (in FP64 every
+    // set must give the reference's per-cell evaluation, bit for bit)
+    const bool face_ok = c->precision != TDT_FP64 && !c->batch;
+    if (ks == "merged" || ks == "split" || ks == "quad" || (ks == "face" && face_ok)) {
+        B->kset = ks == "merged" ? kSetMerged : ks == "split" ? kSetSplit
+                  : ks == "quad" ? kSetQuad : kSetFace;
         B->tuned = true;
     } else if (!ks.empty() && ks.find_first_not_of("0123456789") == std::string::npos) {
         const int v = std::atoi(ks.c_str());
-        if ((v & ~kSetBits) == 0 && (v & kSt1Mask) != kSt1Mask) {
+        if ((v & ~kSetBits) == 0 && ((v & kSt1Mask) != kSt1Face || face_ok)) {
             B->kset = v;
             B->tuned = true;
         }
@@ -205,6 +209,50 @@ int fused_setup(tdt_ctx *c) {
         nrel[v] = cnt <= kNodeWidth
                       ? cnt | su << 4 | sv << 8 | sr << 12 | (pm.novr[4 * v + 3] ? 1 << 16 : 0)
                       : 15;   // more contributors than the records hold: list path
+    }
+    if (face_ok) {
+        // viscous faces: each face once, owned by the lower cell id (a boundary
+        // face by its cell), in owner order; the non-owner finds it by its nodes
+        std::vector<int4> vfl;
+        std::vector<int> cvf((size_t)4 * D.n, -1);
+        vfl.reserve((size_t)2 * D.n + D.g);
+        for (int i = 0; i < D.n; ++i)
+            for (int f = 0; f < 4; ++f) {
+                const int j = pm.nc4[4 * i + f];
+                if (j < D.n && j <= i) continue;
+                cvf[4 * i + f] = (int)vfl.size();
+                vfl.push_back(make_int4(i | f << kTypeShift, nct[4 * i + f], pm.nod4[4 * i + f],
+                                        pm.nod4[4 * i + ((f + 1) & 3)]));
+            }
+        for (int i = 0; i < D.n && ok; ++i)
+            for (int f = 0; f < 4; ++f) {
+                if (cvf[4 * i + f] >= 0) continue;
+                const int j = pm.nc4[4 * i + f];
+                const int b = pm.nod4[4 * i + f], d = pm.nod4[4 * i + ((f + 1) & 3)];
+                int hit = -1;
+                for (int fj = 0; fj < 4 && hit < 0; ++fj) {
+                    const int e = cvf[4 * j + fj];
+                    if (e >= 0 && pm.nc4[4 * j + fj] == i && vfl[e].z == d && vfl[e].w == b)
+                        hit = e;
+                }
+                if (hit < 0) {
+                    ok = false;   // no matching face: the grid is not a conforming quad mesh
+                    break;
+                }
+                cvf[4 * i + f] = hit | (int)0x80000000u;
+            }
+        F.nf = (int)vfl.size();
+        F.nfb8 = ((int)nblocks(F.nf) + 7) & ~7;
+        int4 *d_vfl = dalloc<int4>(*B, vfl.size());
+        int4 *d_cvf = dalloc<int4>(*B, D.n);
+        F.vflux = dalloc<float4>(*B, vfl.size());
+        ok = ok && d_vfl && d_cvf && F.vflux &&
+             hipMemcpy(d_vfl, vfl.data(), vfl.size() * sizeof(int4), hipMemcpyHostToDevice) ==
+                 hipSuccess &&
+             hipMemcpy(d_cvf, cvf.data(), cvf.size() * sizeof(int), hipMemcpyHostToDevice) ==
+                 hipSuccess;
+        F.vfl = d_vfl;
+        F.cvf = d_cvf;
     }
     int *d_nct = dalloc<int>(*B, nct.size());
     int *d_ncon = dalloc<int>(*B, ncon.size());
@@ -322,9 +370,10 @@ static hipError_t autotune(tdt_ctx *c, FusedBufs &B) {
     const size_t qbytes = (size_t)5 * F.nt * sizeof(double);
     const int prof = c->profiling;
     hipError_t e = hipMemcpyAsync(B.qbak, F.qn, qbytes, hipMemcpyDeviceToDevice, c->stream);
-    const int sets[3] = {kSetMerged, kSetSplit, kSetQuad};
-    const int nsets = F.ncases ? 2 : 3;                // a batch has no quad form
-    double t[3][kNumFusedKernels] = {};
+    const int sets[4] = {kSetMerged, kSetSplit, kSetQuad, kSetFace};
+    // a batch has no quad form; the face set is FP32-mixed single cases only
+    const int nsets = F.ncases ? 2 : F.nf > 0 ? 4 : 3;
+    double t[4][kNumFusedKernels] = {};
     for (int k = 0; k < nsets && e == hipSuccess; ++k) {
         e = time_set(c, B, sets[k], kTuneIterations, t[k], nullptr);
         if (e == hipSuccess)
@@ -333,19 +382,21 @@ static hipError_t autotune(tdt_ctx *c, FusedBufs &B) {
     // per slot: pre-pass (merged's k_pre vs quad's kq_pre), stage 1 (merged,
     // split = k_dv + stage 1, quad), stages 2-4 (merged's vs quad's)
     int comp = nsets > 2 && t[2][0] < t[0][0] ? kPreQuad : 0;
-    const double s1[3] = {t[0][2], t[1][1] + t[1][2], t[2][2]};
-    const int s1set[3] = {0, kSt1Split, kSt1Quad};
+    const double s1[4] = {t[0][2], t[1][1] + t[1][2], t[2][2], t[3][1] + t[3][2]};
+    const int s1set[4] = {0, kSt1Split, kSt1Quad, kSt1Face};
     int b1 = 0;
     for (int k = 1; k < nsets; ++k) b1 = s1[k] < s1[b1] ? k : b1;
     comp |= s1set[b1];
     if (nsets > 2 && t[2][3] + t[2][4] + t[2][5] < t[0][3] + t[0][4] + t[0][5]) comp |= kStQuad;
     // whole iterations: the composed set against the uniform ones
-    const int cand[4] = {kSetMerged, kSetSplit, kSetQuad, comp};
+    const int cand[5] = {kSetMerged, kSetSplit, kSetQuad, kSetFace, comp};
     int best = kSetMerged;
     float best_ms = 3.4e38f;
-    for (int k = 0; k < 4 && e == hipSuccess; ++k) {
-        if (k == 2 && nsets < 3) continue;
-        if (k == 3 && (comp == kSetMerged || comp == kSetSplit || comp == kSetQuad)) break;
+    for (int k = 0; k < 5 && e == hipSuccess; ++k) {
+        if ((k == 2 && nsets < 3) || (k == 3 && nsets < 4)) continue;
+        if (k == 4 && (comp == kSetMerged || comp == kSetSplit || comp == kSetQuad ||
+                       comp == kSetFace))
+            break;
         float ms = 0.f;
         e = time_set(c, B, cand[k], kTuneIterations, nullptr, &ms);
         if (e == hipSuccess)
@@ -460,7 +511,8 @@ int fused_iterate(tdt_ctx *c, int niter, double *l2, double *linf, int *first_no
             }
         }
         for (int q = 0; q < kNumFusedKernels; ++q) c->ktimes[q] /= (niter > 0 ? niter : 1);
-        if ((B.kset & kSt1Mask) != kSt1Split) c->ktimes[1] = 0.0;   // no k_dv launch
+        if ((B.kset & kSt1Mask) != kSt1Split && (B.kset & kSt1Mask) != kSt1Face)
+            c->ktimes[1] = 0.0;   // no k_dv / k_vface launch
     } else if (e == hipSuccess) {
         // both graphs are captured on the first call, whatever its length, so
         // that later calls only replay them

@@ -66,6 +66,16 @@ struct FDev {
     int ncases;
     const int4 *cblk, *nblk, *ccase;
     const void *kc64, *kcr;
+    // viscous faces (FP32-mixed kernel sets with kSt1Face): every face once,
+    // vfl[e] = {owner cell | face << kTypeShift, neighbour as in nct, node b,
+    // node d} (an interior face belongs to the lower cell id, a boundary face to
+    // its cell); vflux[e] = the face's VISU, VISV, VIST, VISR terms from the
+    // owner's side; cvf[i] = cell i's four face indices, bit 31 set where i is
+    // not the owner (it applies the terms negated)
+    int nf, nfb8;
+    const int4 *vfl;
+    float4 *vflux;
+    const int4 *cvf;
 };
 
 struct FusedBufs {
@@ -92,6 +102,8 @@ struct FusedBufs {
 constexpr int kPreQuad = 1;        // kq_pre instead of k_pre
 constexpr int kSt1Split = 1 << 1;  // k_dv + k_rk<1, DV=false> instead of k_rk<1> (merged)
 constexpr int kSt1Quad = 2 << 1;   // kq_stage<1>
+constexpr int kSt1Face = 3 << 1;  // FP32-mixed, single case: k_vface (each viscous face
+                                   // once) + k_rk<1> with D inline and VIS gathered
 constexpr int kSt1Mask = 3 << 1;
 constexpr int kStQuad = 1 << 3;    // kq_stage<2..4> instead of k_rk<2..4>
 // FP64 one-lane k_rk (split stage 1, stages 2-4): all four neighbours' states
@@ -100,10 +112,11 @@ constexpr int kStQuad = 1 << 3;    // kq_stage<2..4> instead of k_rk<2..4>
 // small to give a SIMD more than one wavefront anyway.  FP32 always preloads.
 constexpr int kStPreload = 1 << 4;
 constexpr int kSetBits = kPreQuad | kSt1Mask | kStQuad | kStPreload;
-// the three uniform sets
+// the uniform sets
 constexpr int kSetMerged = 0;
 constexpr int kSetSplit = kSt1Split;
 constexpr int kSetQuad = kPreQuad | kSt1Quad | kStQuad;
+constexpr int kSetFace = kSt1Face;
 // stage 4 writes the monitor partials: one per block of its kind
 inline int partial_blocks(const FusedBufs &B, int kset) {
     return (kset & kStQuad) ? B.nbq_cells : B.nb_cells;

@@ -477,6 +477,63 @@ __global__ void __launch_bounds__(kBlock) k_dv(FDev F, Consts<double> K, Consts<
 }
 
 // ---------------------------------------------------------------------------------
+// k_vface: the viscous terms of every face once (viscous.f90:130-217, evaluated
+// from the owning cell's side: A = owner, C = the cell or ghost across, B and D
+// the face's nodes), for the FP32-mixed stage 1 to gather.  The reference
+// evaluates each interior face twice, once from each side; the two agree up to
+// rounding (the gradients and the surface vector change sign together, the
+// four-point averages only by summation order), so the other cell applies the
+// owner's terms negated.  One lane per face, faces in owner-cell order.
+template <class SD, class R, bool B6>
+__global__ void __launch_bounds__(kBlock) k_vface(FDev F, Consts<R> KR) {
+    const int e = (F.remap ? xcd_chunk(blockIdx.x, F.nfb8) : blockIdx.x) * kBlock + threadIdx.x;
+    if (e >= F.nf) return;
+    const int4 fr = F.vfl[e];
+    const int i = fr.x & kIdMask, f = (unsigned)fr.x >> kTypeShift;
+    const int raw = fr.y, bn = fr.z, dn = fr.w, c = raw & kIdMask;
+    const int n = F.n, nt = F.nt, m = F.m;
+    const double *Q = F.qn;
+    const R rhoa = (R)Q[i], c1a = (R)Q[nt + i], c2a = (R)Q[2 * nt + i], c3a = (R)Q[3 * nt + i],
+            c5a = (R)Q[4 * nt + i];
+    R ua, va, ta, pa;
+    uvtp_from_q<R>(rhoa, c1a, c2a, c3a, ua, va, ta, pa, KR);
+    R uc, vc, tc, rhoc, c5c;
+    if (raw < n) {
+        R pc;
+        rhoc = (R)Q[c];
+        uvtp_from_q<R>(rhoc, (R)Q[nt + c], (R)Q[2 * nt + c], (R)Q[3 * nt + c], uc, vc, tc, pc, KR);
+        c5c = (R)Q[4 * nt + c];
+    } else {
+        // the ghost across this boundary face, from the owner (boundary.f90); a
+        // type-3 ghost extrapolates from the neighbour across the opposite face
+        Prim<R> pi, sg;
+        prim_from_q<R>(pi, rhoa, c1a, c2a, c3a, c5a, KR);
+        const int ty = (unsigned)raw >> kTypeShift;
+        R uo = R(0), vo = R(0), to = R(0), po = R(0);
+        if (ty == 3) {
+            const int4 nb = F.nct[i];
+            const int o = sel4((f + 2) & 3, nb.x, nb.y, nb.z, nb.w) & kIdMask;
+            uvtp_from_q<R>((R)Q[o], (R)Q[nt + o], (R)Q[2 * nt + o], (R)Q[3 * nt + o], uo, vo, to,
+                           po, KR);
+        }
+        ghost_state<R>(sg, ty, pi, uo, vo, to, po, (R)ld<SD>(F.gycc, c), KR, g6_for<B6, R>(F, c - n));
+        uc = sg.u; vc = sg.v; tc = sg.t; rhoc = sg.c[0]; c5c = sg.c[4];
+    }
+    const ViscFace<R> o = viscous_face<R>(
+        (R)ld<SD>(F.gsc, (size_t)(2 * f) * n + i), (R)ld<SD>(F.gsc, (size_t)(2 * f + 1) * n + i),
+        (R)ld<SD>(F.gxcc, i), (R)ld<SD>(F.gycc, i), (R)ld<SD>(F.gxn, bn), (R)ld<SD>(F.gyn, bn),
+        (R)ld<SD>(F.gxcc, c), (R)ld<SD>(F.gycc, c), (R)ld<SD>(F.gxn, dn), (R)ld<SD>(F.gyn, dn), ua,
+        va, ta, rhoa, fdiv(c5a, rhoa), (R)ld<SD>(F.node, bn), (R)ld<SD>(F.node, (size_t)m + bn),
+        (R)ld<SD>(F.node, (size_t)2 * m + bn), (R)ld<SD>(F.node, (size_t)3 * m + bn),
+        (R)ld<SD>(F.node, (size_t)4 * m + bn), uc, vc, tc, rhoc, fdiv(c5c, rhoc),
+        (R)ld<SD>(F.node, dn), (R)ld<SD>(F.node, (size_t)m + dn),
+        (R)ld<SD>(F.node, (size_t)2 * m + dn), (R)ld<SD>(F.node, (size_t)3 * m + dn),
+        (R)ld<SD>(F.node, (size_t)4 * m + dn), KR);
+    F.vflux[e] = make_float4((float)(o.u0 + o.u1), (float)(o.v0 + o.v1), (float)(o.t0 + o.t1),
+                             (float)(o.r0 + o.r1));
+}
+
+// ---------------------------------------------------------------------------------
 // k_rk: one RK stage for one cell per lane.  Stage 1 also forms the dissipation D
 // (solver_state.f90:120-223) and viscous fluxes VIS (viscous.f90:62-217) from
 // Q^n and stores them for stages 2-4 (they stay frozen, as in the reference).
@@ -484,11 +541,13 @@ __global__ void __launch_bounds__(kBlock) k_dv(FDev F, Consts<double> K, Consts<
 // built from this cell), the face's flux terms, its viscous terms (stage 1) and
 // its share of the source-term gradient differences.  SI / SO: stage input /
 // output storage types; SD: storage of D, VIS, sensors and nodal averages.
-// DV = false: stage 1 reads D and VIS written by k_dv (the split variant).
-template <class R, class SI, class SO, class SD, int STAGE, bool DV, bool B6, bool BT, bool PL>
+// DV = 0: stage 1 reads D and VIS written by k_dv (the split variant); DV = 2
+// (FP32-mixed): D inline, VIS gathered from the face terms k_vface wrote.
+template <class R, class SI, class SO, class SD, int STAGE, int DV, bool B6, bool BT, bool PL>
 __global__ void __launch_bounds__(kBlock) k_rk(FDev F, Consts<double> K, Consts<R> KR,
                                                double alpha) {
-    constexpr bool kInlineDV = STAGE == 1 && DV;
+    constexpr bool kInlineDV = STAGE == 1 && DV != 0;
+    constexpr bool kFaceVis = STAGE == 1 && DV == 2;
     const int lb = cell_block(F);
     int i, bcount = 0;
     if constexpr (BT) {
@@ -518,6 +577,7 @@ __global__ void __launch_bounds__(kBlock) k_rk(FDev F, Consts<double> K, Consts<
             dti = F.dt[i];
         }
         double dd[5], vs[4];
+        // this cell's four faces' viscous terms (requested before anything waits)
         if (kInlineDV) {
             // dissipation faces; a ghost neighbour is a copy of this cell
             double ci[5];
@@ -627,7 +687,7 @@ __global__ void __launch_bounds__(kBlock) k_rk(FDev F, Consts<double> K, Consts<
             if (f == 1) sb = nbv;
             if (f == 2) src_pair(sa, nbv, sd.xac, sd.yac, sd.qac, sd.rac, sd.uac, sd.vac);
             if (f == 3) src_pair(sb, nbv, sd.xbd, sd.ybd, sd.qbd, sd.rbd, sd.ubd, sd.vbd);
-            if (kInlineDV) {
+            if (kInlineDV && !kFaceVis) {
                 const int4 nd4 = F.nod4[i];
                 const int bn = sel4(f, nd4.x, nd4.y, nd4.z, nd4.w);
                 const int dn = sel4(f, nd4.y, nd4.z, nd4.w, nd4.x);
@@ -646,6 +706,25 @@ __global__ void __launch_bounds__(kBlock) k_rk(FDev F, Consts<double> K, Consts<
                 vv = vv + o.v0 + o.v1;
                 vt = vt + o.t0 + o.t1;
                 vr = vr + o.r0 + o.r1;
+            }
+        }
+        if (kFaceVis) {
+            float4 vfl4[4];
+            int vfsg = 0;
+            const int4 cf = F.cvf[i];
+#pragma unroll
+            for (int f = 0; f < 4; ++f) {
+                const int ef = sel4(f, cf.x, cf.y, cf.z, cf.w);
+                vfl4[f] = F.vflux[ef & 0x7fffffff];
+                vfsg |= ef < 0 ? 1 << f : 0;
+            }
+#pragma unroll
+            for (int f = 0; f < 4; ++f) {
+                const R sg = (vfsg >> f) & 1 ? R(-1) : R(1);
+                vu = vu + sg * (R)vfl4[f].x;
+                vv = vv + sg * (R)vfl4[f].y;
+                vt = vt + sg * (R)vfl4[f].z;
+                vr = vr + sg * (R)vfl4[f].w;
             }
         }
         if (kInlineDV) {
@@ -733,7 +812,7 @@ void launch(tdt_ctx *c, FusedBufs &B, int slot, K kern, unsigned grid, Args... a
 
 // One-lane stage kernel, with the neighbour states preloaded or not (FP64; FP32
 // always preloads).
-template <class R, class SI, class SO, class SD, int STAGE, bool DV, bool B6, bool BT>
+template <class R, class SI, class SO, class SD, int STAGE, int DV, bool B6, bool BT>
 void launch_rk(tdt_ctx *c, FusedBufs &B, int slot, bool preload, const Consts<R> &KR,
                double alpha) {
     if constexpr (sizeof(R) == 8) {
@@ -784,6 +863,13 @@ hipError_t enqueue_iteration(tdt_ctx *c, FusedBufs &B, const Consts<R> &KR) {
         no_dv_slot();
         launch(c, B, 2, kq_stage<R, double, SS, SD, 1, B6>, nq, F, c->K, KR, 0.25);
         break;
+    case kSt1Face:   // FP32-mixed single cases only (fused_setup refuses it otherwise)
+        if constexpr (sizeof(R) == 4 && !BT) {
+            launch(c, B, 1, k_vface<SD, R, B6>, F.nfb8, F, KR);
+            launch(c, B, 2, k_rk<R, double, SS, SD, 1, 2, B6, BT, false>, nbc, F, c->K, KR, 0.25);
+            break;
+        }
+        [[fallthrough]];
     default:
         no_dv_slot();
         launch(c, B, 2, k_rk<R, double, SS, SD, 1, true, B6, BT, false>, nbc, F, c->K, KR, 0.25);

@@ -386,11 +386,12 @@ class Solver:
         self._check(self.L.tdt_kernel_times(self.h, _d(out)))
         return out
 
-    KERNEL_SETS = ("merged", "split", "quad")
-    _UNIFORM = {0: "merged", 2: "split", 13: "quad"}
+    KERNEL_SETS = ("merged", "split", "quad", "face")
+    _UNIFORM = {0: "merged", 2: "split", 13: "quad", 6: "face"}
 
     def kernel_set(self) -> str:
-        """The fused kernel set in use: "merged", "split", "quad", or for a mixed
+        """The fused kernel set in use: "merged", "split", "quad", "face" (FP32-mixed:
+        each viscous face evaluated once), or for a mixed
         choice e.g. "pre=quad,stage1=merged,stages=lane", with "+preload" when the
         FP64 one-lane stage kernels load all neighbour states up front ("" before
         the first tdt_iterate)."""
@@ -403,7 +404,7 @@ class Solver:
         v &= 15
         if v in self._UNIFORM:
             return self._UNIFORM[v] + pl
-        st1 = {0: "merged", 2: "split", 4: "quad"}[v & 6]
+        st1 = {0: "merged", 2: "split", 4: "quad", 6: "face"}[v & 6]
         return (f"pre={'quad' if v & 1 else 'lane'},stage1={st1},"
                 f"stages={'quad' if v & 8 else 'lane'}{pl}")
 

@@ -254,11 +254,15 @@ int tdt_kernel_times(tdt_ctx *ctx, double *times_ms);
  * times the uniform sets kernel by kernel on the current state (restoring it
  * after each trial), composes the fastest choice per slot and keeps whichever
  * of that and the uniform sets runs an iteration fastest, unless the
- * environment variable TDT_KERNELS (merged, split, quad or a set number), read
+ * environment variable TDT_KERNELS (merged, split, quad, face or a set number), read
  * by the first tdt_iterate, pins one. */
 #define TDT_KERNELS_PRE_QUAD 1        /* pre-pass: four lanes per cell (else one)     */
 #define TDT_KERNELS_STAGE1_SPLIT 2    /* dissipation + viscous in a kernel of their own */
 #define TDT_KERNELS_STAGE1_QUAD 4     /* stage 1 (with D and VIS): four lanes per cell */
+#define TDT_KERNELS_STAGE1_FACE 6     /* FP32-mixed single cases: each viscous face
+                                         evaluated once (k_vface, the other cell applies
+                                         it negated), D and the gathered VIS in stage 1;
+                                         equal to the per-cell form to FP32 rounding   */
 #define TDT_KERNELS_STAGES_QUAD 8     /* stages 2-4: four lanes per cell               */
 #define TDT_KERNELS_PRELOAD 16        /* FP64 one-lane stage kernels (split stage 1,
                                          stages 2-4) request all neighbour states up
@@ -267,6 +271,7 @@ int tdt_kernel_times(tdt_ctx *ctx, double *times_ms);
 #define TDT_KERNELS_MERGED 0          /* uniform sets: one lane per cell, D/VIS in stage 1 */
 #define TDT_KERNELS_SPLIT 2
 #define TDT_KERNELS_QUAD 13
+#define TDT_KERNELS_FACE 6
 int tdt_kernel_set(tdt_ctx *ctx, int32_t *set);
 int tdt_synchronize(tdt_ctx *ctx);
 

@@ -134,3 +134,58 @@ def test_batch_refuses_single_case_calls(golden):
         buf = np.zeros(5 * max(b.neles))
         assert b.L.tdt_batch_get_state(b.h, len(cases), lib._d(buf)) == lib.TDT_ERR_ARG
         assert b.L.tdt_batch_get_state(b.h, -1, lib._d(buf)) == lib.TDT_ERR_ARG
+
+
+def sweep_with_fixtures(golden):
+    """The batched sweep at full size (gridgen.sweep_cases(64): 36 channel and 28
+    ramp cases, 2,685,047 cells) followed by two reference fixtures: the 97x49
+    channel as specified (type-7 inlet) and the reference's example case."""
+    from tdt import gridgen
+    cases = [(m, f) for _, m, f in gridgen.sweep_cases(64)]
+    fixtures = ["channel_97x49", "case1"]
+    for name in fixtures:
+        g = golden(name)
+        cases.append((g.mesh, g.flow))
+    return cases, fixtures
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32-mixed"])
+def test_full_sweep_batch(golden, precision, monkeypatch):
+    """The 64-case sweep in one batch context, a few dozen iterations: FP64 every
+    case bit for bit its single-case run (state, L2 and Linf histories);
+    FP32-mixed every case within FP32 rounding of its single-case run (kernels
+    compiled apart).  In FP64 the fixture cases' L2 histories inside the batch
+    agree with the reference's own to 1e-10."""
+    import tdt
+    monkeypatch.setenv("TDT_KERNELS", "merged")
+    cases, fixtures = sweep_with_fixtures(golden)
+    with tdt.Batch(cases, precision) as b:
+        b.cold_start()
+        l2, linf, bad = b.iterate(ITERS)
+        assert bad == 0
+        states = [b.get_state(k) for k in range(len(cases))]
+    worst = 0.0
+    for k, (mesh, flow) in enumerate(cases):
+        st, l2s, linfs = run_single(mesh, flow, precision, ITERS)
+        if precision == "fp64":
+            assert np.array_equal(states[k], st), \
+                f"case {k}: max diff {np.max(np.abs(states[k] - st))}"
+            assert np.array_equal(l2[:, k], l2s), f"case {k}: L2 history"
+            assert np.array_equal(linf[:, k], linfs), f"case {k}: Linf history"
+        else:
+            a = tdt.primitives_from_state(states[k], flow)
+            r = tdt.primitives_from_state(st, flow)
+            for q in ("rho", "p"):
+                err = np.linalg.norm(a[q] - r[q]) / np.linalg.norm(r[q])
+                worst = max(worst, err)
+                assert err < 1e-6, (k, q, err)
+    if precision != "fp64":
+        return   # the L2 of C - COLD is a difference: FP32 rounding shows at 1e-3 there
+    for j, name in enumerate(fixtures):
+        k = len(cases) - len(fixtures) + j
+        ref = golden(name)["hist_l2"].reshape(-1, 5)[:ITERS]
+        dev = float(np.max(np.abs(l2[:, k] - ref) / np.maximum(np.abs(ref), 1e-300)))
+        print(f"{precision}: batch case {k} ({name}) L2 history vs reference: {dev:.2e}")
+        assert dev <= 1e-10, (name, dev)
+    print(f"{precision}: {len(cases)} cases, {sum(m.neles for m, _ in cases)} cells; "
+          f"worst FP32-mixed batch/single deviation {worst:.2e}")

@@ -124,3 +124,67 @@ def test_host_boundary_inputs_files(golden, tmp_path, case, extra):
         den = np.linalg.norm(g["final"][m])
         err = np.linalg.norm(c[m] - g["final"][m]) / (den if den > 0 else 1.0)
         assert err <= 1e-10, f"C{m + 1}: {err:.2e}"
+
+
+def test_host_periodic_restart_and_resume(golden, tmp_path):
+    """The periodic restart file restart<I10>.in (turbnstwod.f90:170-182) is due
+    when NITEROLD + ITER reaches a multiple of 20000.  A run restarted at
+    NITEROLD = 19990 for 20 iterations writes 'restart     20000.in' after its
+    10th iteration and no other; resuming from that file for the last 10
+    iterations (as after a kill) ends on the uninterrupted run's state bit for
+    bit, with NITEROLD = 20010."""
+    from tdt import caseio
+    g = golden("case1")
+    d0, d1, d2 = tmp_path / "warm", tmp_path / "straight", tmp_path / "resumed"
+    for d in (d0, d1, d2):
+        d.mkdir()
+    run_host(d0, dataclasses.replace(g.flow, niter=150, istart=0), g.mesh)
+    c, rtred, npseudo, rinit, _ = caseio.read_restart(str(d0 / "restart.in"), g.mesh.neles)
+    run_host(d1, dataclasses.replace(g.flow, niter=20, istart=1), g.mesh,
+             restart=(c, rtred, npseudo, rinit, 19990))
+    periodic = [f for f in os.listdir(d1) if f.startswith("restart") and f != "restart.in"]
+    assert periodic == ["restart     20000.in"], periodic
+    mid = caseio.read_restart(str(d1 / periodic[0]), g.mesh.neles)
+    assert mid[4] == 20000 and mid[1:4] == (rtred, npseudo, rinit)
+    end = caseio.read_restart(str(d1 / "restart.in"), g.mesh.neles)
+    assert end[4] == 20010
+    run_host(d2, dataclasses.replace(g.flow, niter=10, istart=1), g.mesh, restart=mid)
+    resumed = caseio.read_restart(str(d2 / "restart.in"), g.mesh.neles)
+    assert resumed[4] == 20010
+    assert np.array_equal(resumed[0], end[0])
+
+
+def test_host_stops_on_nonfinite_restart(golden, tmp_path):
+    """A restart.in holding a NaN (the reference's monitor would hide it,
+    turbnstwod.f90:148-157) stops the host with a message and a non-zero exit
+    status instead of running on."""
+    from tdt import caseio
+    g = golden("case1")
+    c = np.array(g["final"], copy=True)
+    c[3, 1234] = np.nan
+    caseio.write_case(str(tmp_path), dataclasses.replace(g.flow, niter=50, istart=1), g.mesh,
+                      restart=(c, 1e-5, 1, 1.5e-5, 200))
+    r = subprocess.run([BIN], cwd=str(tmp_path), capture_output=True, text=True, timeout=300)
+    assert r.returncode != 0
+    assert "tdt: non-finite state at iteration 201" in r.stderr + r.stdout, r.stderr
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(os.path.dirname(__file__), "golden",
+                                                    "duct_385x129_20000.npz")),
+                    reason="duct_385x129_20000 fixture not generated")
+def test_host_periodic_restart_of_a_20000_iteration_run(golden, tmp_path):
+    """The duct over its NITER = 20000 through the host: 'restart     20000.in'
+    holds the reference executable's state (1e-10 relative) and ends with the
+    same RTRED, NPSEUDO, RINIT and NITEROLD lines, printed alike."""
+    from tdt import caseio
+    g = golden("duct_385x129_20000")
+    run_host(tmp_path, dataclasses.replace(g.flow, niter=20000, istart=0), g.mesh, timeout=600)
+    name = str(g["periodic_name"])
+    assert os.path.exists(tmp_path / name), os.listdir(tmp_path)
+    c, *_ = caseio.read_restart(str(tmp_path / name), g.mesh.neles)
+    for m in range(5):
+        err = np.linalg.norm(c[m] - g["final"][m]) / np.linalg.norm(g["final"][m])
+        assert err <= 1e-10, f"C{m + 1}: {err:.2e}"
+    lines = open(tmp_path / name).read().splitlines()
+    assert [ln.strip() for ln in lines[-4:]] == \
+        [ln.strip() for ln in str(g["periodic_tail"]).split("\n")]

@@ -1,0 +1,112 @@
+"""Whole runs of the specified configurations on the GPU against the reference's
+own output at their full iteration counts.
+
+Fixtures (tests/golden/make_golden.py, written by the reference solver):
+  channel_97x49       the 97x49 channel / flat plate as specified (type-7 inlet with
+                      its inputs defined), NITER = 2000, every cell, L2 and Linf
+  duct_385x129_2000   the 385x129 axisymmetric duct, 2000 iterations, every cell
+  duct_385x129_20000  the same duct over its NITER = 20000: L2 history, every cell,
+                      and the reference executable's periodic restart file
+Bar for FP64: per-iteration L2 (and Linf) history and final rho, u, v, p, nu~
+within 1e-10 relative of the reference.  FP32-mixed is held to 1e-5 in pressure
+against the reference's state at the run length.
+"""
+import dataclasses
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+from test_gpu_fused import field_errors
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-10
+
+
+def rel_dev(got, ref):
+    return float(np.max(np.abs(got - ref) / np.maximum(np.abs(ref), 1e-300)))
+
+
+def run(mesh, flow, precision, niter, history=True):
+    import tdt
+    with tdt.Solver(mesh, flow, precision) as s:
+        s.cold_start()
+        l2, linf, bad = s.iterate(niter, history=history)
+        assert bad == 0
+        return s.get_state(), l2, linf
+
+
+def with_minus_one(mesh):
+    """The same grid with NC = -1 on every boundary face (the library resolves
+    them through bc.in's PARENT and SIDE)."""
+    nc = np.array(mesh.nc, copy=True)
+    nc[nc > mesh.neles] = -1
+    return dataclasses.replace(mesh, nc=nc)
+
+
+def check_against_reference(g, state, l2, linf, tol=REL):
+    n = g.nrun
+    dl2 = rel_dev(l2, g["hist_l2"].reshape(-1, 5)[:n])
+    dlinf = rel_dev(linf, g["hist_linf"].reshape(-1, 5)[:n]) if g.has("hist_linf") else 0.0
+    errs = field_errors(state, g["final"], g.flow)
+    print(f"{g.name}: {n} iterations, L2 history {dl2:.2e}, Linf history {dlinf:.2e}, "
+          f"fields {({k: float('%.2e' % v) for k, v in errs.items()})}")
+    assert dl2 <= tol and dlinf <= tol, (dl2, dlinf)
+    assert max(errs.values()) <= tol, errs
+
+
+def test_channel_97x49_against_reference(golden):
+    """Config 0 as specified: 2000 FP64 iterations from the cold start."""
+    g = golden("channel_97x49")
+    st, l2, linf = run(g.mesh, g.flow, "fp64", g.nrun)
+    check_against_reference(g, st, l2, linf)
+
+
+def test_channel_97x49_minus_one_neighbours(golden):
+    """Config 0 with NC = -1 on the boundary faces, as the configuration writes
+    grid.in: the library resolves them through bc.in and must give the ghost-id
+    grid's run bit for bit, so the reference's history too."""
+    g = golden("channel_97x49")
+    m1 = with_minus_one(g.mesh)
+    assert (m1.nc == -1).sum() == g.mesh.nghosts
+    st_ids, l2_ids, linf_ids = run(g.mesh, g.flow, "fp64", g.nrun)
+    st, l2, linf = run(m1, g.flow, "fp64", g.nrun)
+    assert np.array_equal(st, st_ids)
+    assert np.array_equal(l2, l2_ids) and np.array_equal(linf, linf_ids)
+    check_against_reference(g, st, l2, linf)
+
+
+def test_duct_2000_against_reference(golden):
+    """Config 1 over 2000 iterations, every cell (the 40-iteration fixture is
+    sampled)."""
+    g = golden("duct_385x129_2000")
+    st, l2, linf = run(g.mesh, g.flow, "fp64", g.nrun)
+    check_against_reference(g, st, l2, linf)
+
+
+needs_20000 = pytest.mark.skipif(not os.path.exists(os.path.join(GOLDEN, "duct_385x129_20000.npz")),
+                                 reason="duct_385x129_20000 fixture not generated")
+
+
+@needs_20000
+def test_duct_20000_fp64_against_reference(golden):
+    """Config 1 over its NITER = 20000 in FP64: L2 history and final fields."""
+    g = golden("duct_385x129_20000")
+    st, l2, _ = run(g.mesh, g.flow, "fp64", g.nrun)
+    check_against_reference(g, st, l2, None)
+
+
+@needs_20000
+def test_duct_20000_fp32_mixed_against_reference(golden):
+    """FP32-mixed over the duct's 20000 iterations against the reference's state:
+    pressure within 1e-5 relative L2 (the precision target); density, nu~ and
+    the wall shear stress are printed."""
+    from tdt import analysis
+    g = golden("duct_385x129_20000")
+    st, _, _ = run(g.mesh, g.flow, "fp32-mixed", g.nrun, history=False)
+    err = analysis.compare(g.mesh, g.flow, st, g["final"])
+    print("fp32-mixed vs reference after 20000 iterations:",
+          {k: float("%.2e" % v) for k, v in err.items()})
+    assert err["p"] <= 1e-5, err
