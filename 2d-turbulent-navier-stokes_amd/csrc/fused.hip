@@ -62,6 +62,17 @@ __global__ void __launch_bounds__(kBlock) k_narrow(const double *src, float *dst
     if (i < count) dst[i] = (float)src[i];
 }
 
+// node coordinates as pairs (nxy), in the storage type the viscous faces read
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_node_xy(const void *x, const void *y, void *xy,
+                                                     int m) {
+    int v = blockIdx.x * kBlock + threadIdx.x;
+    if (v < m) {
+        static_cast<T *>(xy)[2 * v] = static_cast<const T *>(x)[v];
+        static_cast<T *>(xy)[2 * v + 1] = static_cast<const T *>(y)[v];
+    }
+}
+
 __global__ void __launch_bounds__(kBlock) k_cold_from_qn(FDev F, double *cold) {
     int i = blockIdx.x * kBlock + threadIdx.x;
     if (i < F.n)
@@ -109,21 +120,23 @@ int fused_setup(tdt_ctx *c) {
     bool ok = true;
     if (c->precision == TDT_FP64) {
         // the routine-level arrays double as the fused path's storage
-        F.anu = D.anu; F.anurt = D.anurt; F.d2q = D.d2q; F.dd = D.d24; F.vis = D.vis;
-        F.node = D.nodeavg;
+        // the routine-level D and VIS arrays double as the fused path's
+        F.dd = D.d24; F.vis = D.vis;
+        F.drec = dalloc<double>(*B, (size_t)kRecLen * D.n);
+        F.nrec = dalloc<double>(*B, (size_t)5 * D.m);
+        F.nxy = dalloc<double>(*B, (size_t)2 * D.m);
         F.st[0] = dalloc<double>(*B, (size_t)5 * D.n);
         F.st[1] = dalloc<double>(*B, (size_t)5 * D.n);
-        ok = F.st[0] && F.st[1];
+        ok = F.drec && F.nrec && F.nxy && F.st[0] && F.st[1];
     } else {
-        F.anu = dalloc<float>(*B, D.n);
-        F.anurt = dalloc<float>(*B, D.n);
-        F.d2q = dalloc<float>(*B, (size_t)5 * D.nt);
+        F.drec = dalloc<float>(*B, (size_t)kRecLen * D.n);
+        F.nrec = dalloc<float>(*B, (size_t)5 * D.m);
+        F.nxy = dalloc<float>(*B, (size_t)2 * D.m);
         F.dd = dalloc<float>(*B, (size_t)5 * D.n);
         F.vis = dalloc<float>(*B, (size_t)4 * D.n);
-        F.node = dalloc<float>(*B, (size_t)5 * D.m);
         F.st[0] = dalloc<float>(*B, (size_t)5 * D.n);
         F.st[1] = dalloc<float>(*B, (size_t)5 * D.n);
-        ok = F.anu && F.anurt && F.d2q && F.dd && F.vis && F.node && F.st[0] && F.st[1];
+        ok = F.drec && F.nrec && F.nxy && F.dd && F.vis && F.st[0] && F.st[1];
         F.gxcc = dalloc<float>(*B, D.nt);
         F.gycc = dalloc<float>(*B, D.nt);
         F.gvol = dalloc<float>(*B, D.nt);
@@ -428,15 +441,24 @@ int fused_iterate(tdt_ctx *c, int niter, double *l2, double *linf, int *first_no
     if (rc != TDT_OK) return rc;
     FusedBufs &B = *c->fused;
     FDev &F = B.F;
-    if (c->precision != TDT_FP64 && B.geom_version != c->geom_version) {
-        // FP32 copies of the geometry the flux arithmetic reads
+    if (B.geom_version != c->geom_version) {
+        // FP32 copies of the geometry the flux arithmetic reads, and the node
+        // coordinate pairs the viscous faces read
         const Dev &D = c->D;
+        const bool fp64 = c->precision == TDT_FP64;
         const struct { const double *s; const void *d; int cnt; } cp[] = {
             {D.xcc, F.gxcc, D.nt}, {D.ycc, F.gycc, D.nt}, {D.vol, F.gvol, D.nt},
             {D.sc, F.gsc, 8 * D.n}, {D.xn, F.gxn, D.m}, {D.yn, F.gyn, D.m}};
         for (const auto &x : cp)
-            hipLaunchKernelGGL(k_narrow, dim3(nblocks(x.cnt)), dim3(kBlock), 0, c->stream, x.s,
-                               (float *)x.d, x.cnt);
+            if (!fp64)
+                hipLaunchKernelGGL(k_narrow, dim3(nblocks(x.cnt)), dim3(kBlock), 0, c->stream,
+                                   x.s, (float *)x.d, x.cnt);
+        if (fp64)
+            hipLaunchKernelGGL(k_node_xy<double>, dim3(nblocks(D.m)), dim3(kBlock), 0, c->stream,
+                               F.gxn, F.gyn, F.nxy, D.m);
+        else
+            hipLaunchKernelGGL(k_node_xy<float>, dim3(nblocks(D.m)), dim3(kBlock), 0, c->stream,
+                               F.gxn, F.gyn, F.nxy, D.m);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) {
             c->err = std::string("geometry copies: ") + hipGetErrorString(e);

@@ -22,6 +22,9 @@ constexpr int kTypeShift = 28;
 constexpr int kIdMask = (1 << kTypeShift) - 1;
 // Node gathers of up to this many contributors use fixed-width records.
 constexpr int kNodeWidth = 4;
+// Values per cell record (drec) and their slots.
+constexpr int kRecLen = 8;
+constexpr int kRecAnu = 0, kRecAnurt = 1, kRecVbydt = 2, kRecD2q = 3;
 
 struct FDev {
     int n, g, m, nt;
@@ -49,11 +52,16 @@ struct FDev {
     double *qn;                     // [5][nt] Q^n (the C array), FP64
     void *st[2];                    // [5][n] stage states Q(1), Q(2)/Q(3)
     double *dt;                     // [n]
-    void *anu, *anurt;              // [n]  (storage type SD)
-    void *d2q;                      // [5][nt]
+    // drec[n]: per-iteration cell records of eight values of the storage type
+    // SD, ANU, ANURT, VOL/DT (VOL when LOCAL = 0), D2Q1..D2Q5 of Q^n, in planes
+    // of 16-byte vectors (a gather of one costs two or four 16-byte loads)
+    void *drec;
+    // nodal averages UN, VN, TN, RHON as 16-byte vectors (one plane in FP32,
+    // two in FP64), then RN: 5 * m values of SD; nxy[m]: the node coordinates
+    // as SD pairs (float2 / double2), refreshed when the geometry changes
+    void *nrec, *nxy;
     void *dd;                       // [5][n] dissipation D
     void *vis;                      // [4][n]
-    void *node;                     // [5][m] UN VN TN RHON RN
     unsigned long long *dtslot;     // DTMIN bits for LOCAL = 0
     double *part;                   // per-block monitor partials
     double *hist;                   // [niter][10]

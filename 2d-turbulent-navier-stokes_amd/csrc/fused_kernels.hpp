@@ -40,6 +40,142 @@ __device__ __forceinline__ void stv(void *p, size_t i, double v) {
     static_cast<T *>(p)[i] = (T)v;
 }
 
+// Records (drec, nrec): kRecLen values of type T per cell or node, stored as
+// planes of 16-byte vectors: plane k holds values 4k..4k+3 (FP32) or 2k, 2k+1
+// (FP64) of every record, plane stride = the record count.  A wave's record
+// store of one plane is one contiguous 1 KB run, and a gather of a record
+// costs two (FP32) or four (FP64) 16-byte loads.
+template <class T>
+__device__ __forceinline__ void rec_get(const void *p, size_t stride, size_t i,
+                                        T (&r)[kRecLen]) {
+    if constexpr (sizeof(T) == 4) {
+        const float4 *q = static_cast<const float4 *>(p);
+        const float4 a = q[i], b = q[stride + i];
+        r[0] = a.x; r[1] = a.y; r[2] = a.z; r[3] = a.w;
+        r[4] = b.x; r[5] = b.y; r[6] = b.z; r[7] = b.w;
+    } else {
+        const double2 *q = static_cast<const double2 *>(p);
+        const double2 a = q[i], b = q[stride + i], c = q[2 * stride + i], d = q[3 * stride + i];
+        r[0] = a.x; r[1] = a.y; r[2] = b.x; r[3] = b.y;
+        r[4] = c.x; r[5] = c.y; r[6] = d.x; r[7] = d.y;
+    }
+}
+template <class T>
+__device__ __forceinline__ void rec_put(void *p, size_t stride, size_t i, const T (&r)[kRecLen]) {
+    if constexpr (sizeof(T) == 4) {
+        float4 *q = static_cast<float4 *>(p);
+        q[i] = make_float4(r[0], r[1], r[2], r[3]);
+        q[stride + i] = make_float4(r[4], r[5], r[6], r[7]);
+    } else {
+        double2 *q = static_cast<double2 *>(p);
+        q[i] = make_double2(r[0], r[1]);
+        q[stride + i] = make_double2(r[2], r[3]);
+        q[2 * stride + i] = make_double2(r[4], r[5]);
+        q[3 * stride + i] = make_double2(r[6], r[7]);
+    }
+}
+// Value `slot` of record i (single-value stores of the quad pre-pass).
+template <class T>
+__device__ __forceinline__ T &rec_slot(void *p, size_t stride, size_t i, int slot) {
+    constexpr int w = 16 / sizeof(T);   // values per 16-byte vector
+    return static_cast<T *>(p)[((size_t)(slot / w) * stride + i) * w + slot % w];
+}
+
+// Node averages (UN, VN, TN, RHON, RN) of node v: slots 0..3 in 16-byte
+// vector planes, slot 4 in a plain array after them.
+template <class T>
+__device__ __forceinline__ T &nav_slot(void *p, size_t m, size_t v, int slot) {
+    if (slot == 4) return static_cast<T *>(p)[4 * m + v];
+    constexpr int w = 16 / sizeof(T);
+    return static_cast<T *>(p)[((size_t)(slot / w) * m + v) * w + slot % w];
+}
+template <class T>
+__device__ __forceinline__ void nav_put(void *p, size_t m, size_t v, const T (&a)[5]) {
+    if constexpr (sizeof(T) == 4) {
+        static_cast<float4 *>(p)[v] = make_float4(a[0], a[1], a[2], a[3]);
+    } else {
+        static_cast<double2 *>(p)[v] = make_double2(a[0], a[1]);
+        static_cast<double2 *>(p)[m + v] = make_double2(a[2], a[3]);
+    }
+    static_cast<T *>(p)[4 * m + v] = a[4];
+}
+// ... and with the node's coordinates
+template <class T>
+__device__ __forceinline__ void nav_get(const FDev &F, size_t v, T (&a)[5], T &x, T &y) {
+    const size_t m = F.m;
+    if constexpr (sizeof(T) == 4) {
+        const float4 q = static_cast<const float4 *>(F.nrec)[v];
+        a[0] = q.x; a[1] = q.y; a[2] = q.z; a[3] = q.w;
+        const float2 c = static_cast<const float2 *>(F.nxy)[v];
+        x = c.x; y = c.y;
+    } else {
+        const double2 q0 = static_cast<const double2 *>(F.nrec)[v];
+        const double2 q1 = static_cast<const double2 *>(F.nrec)[m + v];
+        a[0] = q0.x; a[1] = q0.y; a[2] = q1.x; a[3] = q1.y;
+        const double2 c = static_cast<const double2 *>(F.nxy)[v];
+        x = c.x; y = c.y;
+    }
+    a[4] = static_cast<const T *>(F.nrec)[4 * m + v];
+}
+
+// VOL/DT of a cell from its record slot: the pre-pass stores VOL/DT when each
+// cell has its own time step, VOL when all share DTMIN (known only after the
+// pre-pass), here dtmin.
+template <class R, class SD>
+__device__ __forceinline__ R rec_vbydt(const SD (&r)[kRecLen], int local, double dtmin) {
+    return local ? (R)r[kRecVbydt] : fdiv((R)r[kRecVbydt], (R)dtmin);
+}
+
+// The dissipation D of cell i (solver_state.f90:183-223) from the records of i
+// and its four face neighbours (a ghost neighbour is a copy of the cell) and
+// the FP64 states: returns D1..D5 in d (2nd-difference minus 4th-difference
+// sums, each accumulated in face order).
+template <class R, class SD>
+__device__ __forceinline__ void dissipation_rec(const FDev &F, int i, const int (&nb)[4],
+                                                const double (&ci)[5], int local, double dti,
+                                                R (&d)[5]) {
+    const int nt = F.nt, n = F.n;
+    const double *Q = F.qn;
+    SD ri[kRecLen];
+    rec_get<SD>(F.drec, F.n, i, ri);
+    const R volbydt = rec_vbydt<R>(ri, local, dti);
+    R d2[5], d4[5];
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+        const int j = nb[f] < n ? nb[f] : i;
+        SD rj[kRecLen];
+        rec_get<SD>(F.drec, F.n, j, rj);
+        const R vbn = rec_vbydt<R>(rj, local, dti);
+        R a2, a4, b2, b4;
+        jst_coeffs_v<R>((R)ri[kRecAnu], (R)rj[kRecAnu], volbydt, vbn, a2, a4);
+        jst_coeffs_v<R>((R)ri[kRecAnurt], (R)rj[kRecAnurt], volbydt, vbn, b2, b4);
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+            const R t2 = (q < 4 ? a2 : b2) * (R)(Q[q * nt + j] - ci[q]);
+            const R t4 = (q < 4 ? a4 : b4) * ((R)rj[kRecD2q + q] - (R)ri[kRecD2q + q]);
+            d2[q] = f == 0 ? t2 : d2[q] + t2;
+            d4[q] = f == 0 ? t4 : d4[q] + t4;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 5; ++q) d[q] = d2[q] - d4[q];
+}
+
+// The viscous terms of face f of cell A (viscous.f90:130-217): B, D = the
+// face's nodes (their records: averages and coordinates), C = the neighbour.
+template <class R, class SD>
+__device__ __forceinline__ ViscFace<R> viscous_face_rec(const FDev &F, R sx, R sy, R xa, R ya,
+                                                        R ua, R va, R ta, R rhoa, R ra, int bn,
+                                                        int dn, R xc, R yc, R uc, R vc, R tc,
+                                                        R rhoc, R rc, const Consts<R> &KR) {
+    SD b[5], e[5], xb, yb, xd, yd;
+    nav_get<SD>(F, bn, b, xb, yb);
+    nav_get<SD>(F, dn, e, xd, yd);
+    return viscous_face<R>(sx, sy, xa, ya, (R)xb, (R)yb, xc, yc, (R)xd, (R)yd, ua, va, ta, rhoa,
+                           ra, (R)b[0], (R)b[1], (R)b[2], (R)b[3], (R)b[4], uc, vc, tc, rhoc, rc,
+                           (R)e[0], (R)e[1], (R)e[2], (R)e[3], (R)e[4], KR);
+}
+
 // Folds the pending stage-4 partials of the previous iteration into the
 // history (one block; block 0 of k_pre and k_finalize call it).  The kernel
 // boundary orders the partial writes before these reads.
@@ -214,10 +350,12 @@ __device__ __forceinline__ void node_gather_list(const FDev &F, const Consts<R> 
 }
 
 // ---------------------------------------------------------------------------------
-// k_pre: cell blocks [0, ncb8), node blocks [ncb8, ncb8 + nnb8) (both padded
-// to multiples of 8 and spread so that each XCD works on contiguous cells and
-// the nodes around them), and a last block of its own that folds the previous
-// iteration's monitor partials (off every cell's path).
+// k_pre: ncb8 cell blocks and nnb8 node blocks (both padded to multiples of 8
+// and spread so that each XCD works on contiguous cells and the nodes around
+// them, alternating between the two), and a last block of its own that folds
+// the previous iteration's monitor partials (off every cell's path).  Cells:
+// primitives of Q^n, DT, and the dissipation record (sensors, VOL/DT, D2Q);
+// nodes: the nodal averages.
 template <class SD, class R, bool B6, bool BT>
 __global__ void __launch_bounds__(kBlock) k_pre(FDev F, Consts<double> K, Consts<R> KR,
                                                 int nb_partials, int ncb8, int nnb8) {
@@ -232,8 +370,27 @@ __global__ void __launch_bounds__(kBlock) k_pre(FDev F, Consts<double> K, Consts
         fold_monitor(F, nb_partials);
         return;
     }
-    if (b < ncb8) {
-        const int cb = F.remap ? xcd_chunk(b, ncb8) : b;
+    // Each XCD takes its eighth of the cell blocks and of the node blocks,
+    // alternating between them, so that a node block runs while the states of
+    // the cells around its nodes are still in that XCD's L2.
+    bool cells;
+    int lcb;
+    if (F.remap) {
+        const int xcd = b & 7, l = b >> 3, cx = ncb8 >> 3, nx = nnb8 >> 3;
+        const int both = 2 * min(cx, nx);
+        if (l < both) {
+            cells = (l & 1) == 0;
+            lcb = xcd * (cells ? cx : nx) + (l >> 1);
+        } else {
+            cells = cx > nx;
+            lcb = xcd * (cells ? cx : nx) + (l - both + both / 2);
+        }
+    } else {
+        cells = b < ncb8;
+        lcb = cells ? b : b - ncb8;
+    }
+    if (cells) {
+        const int cb = lcb;
         int i;
         bool act;
         if constexpr (BT) {
@@ -274,21 +431,26 @@ __global__ void __launch_bounds__(kBlock) k_pre(FDev F, Consts<double> K, Consts
             }
             double anun = fabs(pn[0] - p) + fabs(pn[1] - p) + fabs(pn[2] - p) + fabs(pn[3] - p);
             double anud = fabs(pn[0] + p) + fabs(pn[1] + p) + fabs(pn[2] + p) + fabs(pn[3] + p);
-            stv<SD>(F.anu, i, fdiv(anun, anud));
+            SD rec[kRecLen];
+            rec[kRecAnu] = (SD)fdiv(anun, anud);
             anun = fabs(cn[4][0] - c4) + fabs(cn[4][1] - c4) + fabs(cn[4][2] - c4) +
                    fabs(cn[4][3] - c4);
             anud = fabs(cn[4][0] + c4) + fabs(cn[4][1] + c4) + fabs(cn[4][2] + c4) +
                    fabs(cn[4][3] + c4);
-            stv<SD>(F.anurt, i, fdiv(anun, dmax(anud, 8.0 * c0 * K.rtol)));
+            rec[kRecAnurt] = (SD)fdiv(anun, dmax(anud, 8.0 * c0 * K.rtol));
+            // VOL/DT as the dissipation faces divide them (R arithmetic), or VOL
+            // when the time step is the global minimum
+            const R vol = (R)ld<SD>(F.gvol, i);
+            rec[kRecVbydt] = (SD)(K.local ? fdiv(vol, (R)dt) : vol);
             const double ci[5] = {c0, c1, c2, c3, c4};
             for (int m = 0; m < 5; ++m)
-                stv<SD>(F.d2q, (size_t)m * nt + i,
-                        cn[m][0] + cn[m][1] + cn[m][2] + cn[m][3] - 4.0 * ci[m]);
+                rec[kRecD2q + m] = (SD)(cn[m][0] + cn[m][1] + cn[m][2] + cn[m][3] - 4.0 * ci[m]);
+            rec_put<SD>(F.drec, F.n, i, rec);
         }
         if (!K.local) block_atomic_min_pos(dtl, reinterpret_cast<double *>(F.dtslot));
         return;
     }
-    const int nbl = F.remap ? xcd_chunk(b - ncb8, nnb8) : b - ncb8;
+    const int nbl = lcb;
     int v;
     if constexpr (BT) {
         const int4 bl = F.nblk[nbl];
@@ -355,14 +517,11 @@ __global__ void __launch_bounds__(kBlock) k_pre(FDev F, Consts<double> K, Consts
         ar = F.novr[v].w ? KR.rtol_rtred : R(0);
         node_gather_list<R, B6>(F, KR, v, au, av, at, arho, ar);
     }
-    const int m = F.m;
     const R nc = (R)(cnt <= kNodeWidth ? cnt : F.nptr[v + 1] - F.nptr[v]);
     const Recip<R> rcnt(nc);
-    stv<SD>(F.node, v, rcnt.div(au));
-    stv<SD>(F.node, (size_t)m + v, rcnt.div(av));
-    stv<SD>(F.node, (size_t)2 * m + v, rcnt.div(at));
-    stv<SD>(F.node, (size_t)3 * m + v, rcnt.div(arho));
-    stv<SD>(F.node, (size_t)4 * m + v, dmax(KR.rtol, rcnt.div(ar)));
+    const SD a[5] = {(SD)rcnt.div(au), (SD)rcnt.div(av), (SD)rcnt.div(at), (SD)rcnt.div(arho),
+                     (SD)dmax(KR.rtol, rcnt.div(ar))};
+    nav_put<SD>(F.nrec, F.m, v, a);
 }
 
 // ---------------------------------------------------------------------------------
@@ -380,7 +539,7 @@ __global__ void __launch_bounds__(kBlock) k_dv(FDev F, Consts<double> K, Consts<
         i = cell_block(F) * kBlock + threadIdx.x;
         if (i >= F.n) return;
     }
-    const int nt = F.nt, n = F.n, m = F.m;
+    const int nt = F.nt, n = F.n;
     const double *Q = F.qn;
     double dti = K.local ? F.dt[i] : __longlong_as_double((long long)*F.dtslot);
     if (!K.local) F.dt[i] = dti;
@@ -391,33 +550,12 @@ __global__ void __launch_bounds__(kBlock) k_dv(FDev F, Consts<double> K, Consts<
     // ---- dissipation (solver_state.f90:183-223); ghost neighbour = parent copy
     double ci[5];
     for (int q = 0; q < 5; ++q) ci[q] = Q[q * nt + i];
-    R voli = (R)ld<SD>(F.gvol, i);
-    R volbydt = fdiv(voli, (R)dti);
-    R anui = (R)ld<SD>(F.anu, i), anurti = (R)ld<SD>(F.anurt, i);
-    R d2qi[5];
-    for (int q = 0; q < 5; ++q) d2qi[q] = (R)ld<SD>(F.d2q, (size_t)q * nt + i);
-    R d2[5], d4[5];
-    for (int k = 0; k < 4; ++k) {
-        int j = raw[k] < n ? raw[k] : i;
-        R dtj = K.local ? (R)F.dt[j] : (R)dti;
-        R volj = (R)ld<SD>(F.gvol, j);
-        R a2, a4, b2, b4;
-        jst_coeffs<R>(anui, (R)ld<SD>(F.anu, j), volbydt, volj, dtj, a2, a4);
-        jst_coeffs<R>(anurti, (R)ld<SD>(F.anurt, j), volbydt, volj, dtj, b2, b4);
-        for (int q = 0; q < 5; ++q) {
-            R c2 = q < 4 ? a2 : b2, c4 = q < 4 ? a4 : b4;
-            R dc = (R)(Q[q * nt + j] - ci[q]);
-            R dq = (R)ld<SD>(F.d2q, (size_t)q * nt + j) - d2qi[q];
-            if (k == 0) {
-                d2[q] = c2 * dc;
-                d4[q] = c4 * dq;
-            } else {
-                d2[q] = d2[q] + c2 * dc;
-                d4[q] = d4[q] + c4 * dq;
-            }
-        }
+    {
+        const int nbr[4] = {raw[0] & kIdMask, raw[1] & kIdMask, raw[2] & kIdMask, raw[3] & kIdMask};
+        R dsum[5];
+        dissipation_rec<R, SD>(F, i, nbr, ci, K.local, dti, dsum);
+        for (int q = 0; q < 5; ++q) stv<SD>(F.dd, (size_t)q * n + i, dsum[q]);
     }
-    for (int q = 0; q < 5; ++q) stv<SD>(F.dd, (size_t)q * n + i, d2[q] - d4[q]);
     // ---- viscous fluxes (viscous.f90:62-70, 130-217)
     R ua, va, ta, pa;
     R rhoa = (R)ci[0];
@@ -455,16 +593,10 @@ __global__ void __launch_bounds__(kBlock) k_dv(FDev F, Consts<double> K, Consts<
             }
         }
         R rc = fdiv(c5c, rhoc);
-        ViscFace<R> o = viscous_face<R>(
-            (R)ld<SD>(F.gsc, (2 * f) * n + i), (R)ld<SD>(F.gsc, (2 * f + 1) * n + i), xa, ya,
-            (R)ld<SD>(F.gxn, bnode), (R)ld<SD>(F.gyn, bnode), (R)ld<SD>(F.gxcc, c),
-            (R)ld<SD>(F.gycc, c), (R)ld<SD>(F.gxn, dnode), (R)ld<SD>(F.gyn, dnode), ua, va, ta,
-            rhoa, ra, (R)ld<SD>(F.node, bnode), (R)ld<SD>(F.node, (size_t)m + bnode),
-            (R)ld<SD>(F.node, (size_t)2 * m + bnode), (R)ld<SD>(F.node, (size_t)3 * m + bnode),
-            (R)ld<SD>(F.node, (size_t)4 * m + bnode), uc, vc, tc, rhoc, rc,
-            (R)ld<SD>(F.node, dnode), (R)ld<SD>(F.node, (size_t)m + dnode),
-            (R)ld<SD>(F.node, (size_t)2 * m + dnode), (R)ld<SD>(F.node, (size_t)3 * m + dnode),
-            (R)ld<SD>(F.node, (size_t)4 * m + dnode), KR);
+        ViscFace<R> o = viscous_face_rec<R, SD>(
+            F, (R)ld<SD>(F.gsc, (2 * f) * n + i), (R)ld<SD>(F.gsc, (2 * f + 1) * n + i), xa, ya,
+            ua, va, ta, rhoa, ra, bnode, dnode, (R)ld<SD>(F.gxcc, c), (R)ld<SD>(F.gycc, c), uc,
+            vc, tc, rhoc, rc, KR);
         vu = vu + o.u0 + o.u1;
         vv = vv + o.v0 + o.v1;
         vt = vt + o.t0 + o.t1;
@@ -491,7 +623,7 @@ __global__ void __launch_bounds__(kBlock) k_vface(FDev F, Consts<R> KR) {
     const int4 fr = F.vfl[e];
     const int i = fr.x & kIdMask, f = (unsigned)fr.x >> kTypeShift;
     const int raw = fr.y, bn = fr.z, dn = fr.w, c = raw & kIdMask;
-    const int n = F.n, nt = F.nt, m = F.m;
+    const int n = F.n, nt = F.nt;
     const double *Q = F.qn;
     const R rhoa = (R)Q[i], c1a = (R)Q[nt + i], c2a = (R)Q[2 * nt + i], c3a = (R)Q[3 * nt + i],
             c5a = (R)Q[4 * nt + i];
@@ -519,16 +651,10 @@ __global__ void __launch_bounds__(kBlock) k_vface(FDev F, Consts<R> KR) {
         ghost_state<R>(sg, ty, pi, uo, vo, to, po, (R)ld<SD>(F.gycc, c), KR, g6_for<B6, R>(F, c - n));
         uc = sg.u; vc = sg.v; tc = sg.t; rhoc = sg.c[0]; c5c = sg.c[4];
     }
-    const ViscFace<R> o = viscous_face<R>(
-        (R)ld<SD>(F.gsc, (size_t)(2 * f) * n + i), (R)ld<SD>(F.gsc, (size_t)(2 * f + 1) * n + i),
-        (R)ld<SD>(F.gxcc, i), (R)ld<SD>(F.gycc, i), (R)ld<SD>(F.gxn, bn), (R)ld<SD>(F.gyn, bn),
-        (R)ld<SD>(F.gxcc, c), (R)ld<SD>(F.gycc, c), (R)ld<SD>(F.gxn, dn), (R)ld<SD>(F.gyn, dn), ua,
-        va, ta, rhoa, fdiv(c5a, rhoa), (R)ld<SD>(F.node, bn), (R)ld<SD>(F.node, (size_t)m + bn),
-        (R)ld<SD>(F.node, (size_t)2 * m + bn), (R)ld<SD>(F.node, (size_t)3 * m + bn),
-        (R)ld<SD>(F.node, (size_t)4 * m + bn), uc, vc, tc, rhoc, fdiv(c5c, rhoc),
-        (R)ld<SD>(F.node, dn), (R)ld<SD>(F.node, (size_t)m + dn),
-        (R)ld<SD>(F.node, (size_t)2 * m + dn), (R)ld<SD>(F.node, (size_t)3 * m + dn),
-        (R)ld<SD>(F.node, (size_t)4 * m + dn), KR);
+    const ViscFace<R> o = viscous_face_rec<R, SD>(
+        F, (R)ld<SD>(F.gsc, (size_t)(2 * f) * n + i), (R)ld<SD>(F.gsc, (size_t)(2 * f + 1) * n + i),
+        (R)ld<SD>(F.gxcc, i), (R)ld<SD>(F.gycc, i), ua, va, ta, rhoa, fdiv(c5a, rhoa), bn, dn,
+        (R)ld<SD>(F.gxcc, c), (R)ld<SD>(F.gycc, c), uc, vc, tc, rhoc, fdiv(c5c, rhoc), KR);
     F.vflux[e] = make_float4((float)(o.u0 + o.u1), (float)(o.v0 + o.v1), (float)(o.t0 + o.t1),
                              (float)(o.r0 + o.r1));
 }
@@ -581,35 +707,14 @@ __global__ void __launch_bounds__(kBlock) k_rk(FDev F, Consts<double> K, Consts<
         if (kInlineDV) {
             // dissipation faces; a ghost neighbour is a copy of this cell
             double ci[5];
-            R d2qi[5];
+#pragma unroll
+            for (int q = 0; q < 5; ++q) ci[q] = Q[q * nt + i];
+            const int nbr[4] = {nb4.x, nb4.y, nb4.z, nb4.w};
+            R dsum[5];
+            dissipation_rec<R, SD>(F, i, nbr, ci, K.local, dti, dsum);
 #pragma unroll
             for (int q = 0; q < 5; ++q) {
-                ci[q] = Q[q * nt + i];
-                d2qi[q] = (R)ld<SD>(F.d2q, (size_t)q * nt + i);
-            }
-            const R anui = (R)ld<SD>(F.anu, i), anurti = (R)ld<SD>(F.anurt, i);
-            const R volbydt = fdiv((R)ld<SD>(F.gvol, i), (R)dti);
-            R d2[5], d4[5];
-#pragma unroll
-            for (int f = 0; f < 4; ++f) {
-                const int jn = sel4(f, nb4.x, nb4.y, nb4.z, nb4.w);
-                const int j = jn < n ? jn : i;
-                const R dtj = K.local ? (R)F.dt[j] : (R)dti;
-                const R volj = (R)ld<SD>(F.gvol, j);
-                R a2, a4, b2, b4;
-                jst_coeffs<R>(anui, (R)ld<SD>(F.anu, j), volbydt, volj, dtj, a2, a4);
-                jst_coeffs<R>(anurti, (R)ld<SD>(F.anurt, j), volbydt, volj, dtj, b2, b4);
-#pragma unroll
-                for (int q = 0; q < 5; ++q) {
-                    const R t2 = (q < 4 ? a2 : b2) * (R)(Q[q * nt + j] - ci[q]);
-                    const R t4 = (q < 4 ? a4 : b4) * ((R)ld<SD>(F.d2q, (size_t)q * nt + j) - d2qi[q]);
-                    d2[q] = f == 0 ? t2 : d2[q] + t2;
-                    d4[q] = f == 0 ? t4 : d4[q] + t4;
-                }
-            }
-#pragma unroll
-            for (int q = 0; q < 5; ++q) {
-                dd[q] = (double)(SD)(d2[q] - d4[q]);
+                dd[q] = (double)(SD)dsum[q];
                 stv<SD>(F.dd, (size_t)q * n + i, dd[q]);
             }
         } else {
@@ -691,17 +796,10 @@ __global__ void __launch_bounds__(kBlock) k_rk(FDev F, Consts<double> K, Consts<
                 const int4 nd4 = F.nod4[i];
                 const int bn = sel4(f, nd4.x, nd4.y, nd4.z, nd4.w);
                 const int dn = sel4(f, nd4.y, nd4.z, nd4.w, nd4.x);
-                const int m = F.m;
-                const ViscFace<R> o = viscous_face<R>(
-                    sx, sy, (R)ld<SD>(F.gxcc, i), (R)ld<SD>(F.gycc, i), (R)ld<SD>(F.gxn, bn),
-                    (R)ld<SD>(F.gyn, bn), xj, yj, (R)ld<SD>(F.gxn, dn), (R)ld<SD>(F.gyn, dn),
-                    pi.u, pi.v, pi.t, pi.c[0], fdiv(pi.c[4], pi.c[0]),
-                    (R)ld<SD>(F.node, bn), (R)ld<SD>(F.node, (size_t)m + bn),
-                    (R)ld<SD>(F.node, (size_t)2 * m + bn), (R)ld<SD>(F.node, (size_t)3 * m + bn),
-                    (R)ld<SD>(F.node, (size_t)4 * m + bn), pj.u, pj.v, pj.t, pj.c[0],
-                    fdiv(pj.c[4], pj.c[0]), (R)ld<SD>(F.node, dn), (R)ld<SD>(F.node, (size_t)m + dn),
-                    (R)ld<SD>(F.node, (size_t)2 * m + dn), (R)ld<SD>(F.node, (size_t)3 * m + dn),
-                    (R)ld<SD>(F.node, (size_t)4 * m + dn), KR);
+                const ViscFace<R> o = viscous_face_rec<R, SD>(
+                    F, sx, sy, (R)ld<SD>(F.gxcc, i), (R)ld<SD>(F.gycc, i), pi.u, pi.v, pi.t,
+                    pi.c[0], fdiv(pi.c[4], pi.c[0]), bn, dn, xj, yj, pj.u, pj.v, pj.t, pj.c[0],
+                    fdiv(pj.c[4], pj.c[0]), KR);
                 vu = vu + o.u0 + o.u1;
                 vv = vv + o.v0 + o.v1;
                 vt = vt + o.t0 + o.t1;

@@ -465,6 +465,17 @@ __device__ __forceinline__ R source_terms(const SrcNbr<R> &a, const SrcNbr<R> &b
     return source_terms_d(d, rho, c5, t, p, v, ycc, src3, K);
 }
 
+// The same with the neighbour's VOL/DT given (vbn = VOL(n)/DT(n), the very
+// division jst_coeffs performs).
+template <class R>
+__device__ __forceinline__ void jst_coeffs_v(R si, R sn, R volbydt, R vbn, R &d2n, R &d4n) {
+    R e2n = R(kC02) * dmax(si, sn);
+    R e4n = dmax(R(0), R(kC04) - e2n);
+    R d24n = R(0.5) * (volbydt + vbn);
+    d2n = e2n * d24n;
+    d4n = e4n * d24n;
+}
+
 // Dissipation face coefficients (solver_state.f90:192-194): returns D2N, D4N.
 template <class R>
 __device__ __forceinline__ void jst_coeffs(R si, R sn, R volbydt, R vol_n, R dt_n, R &d2n,

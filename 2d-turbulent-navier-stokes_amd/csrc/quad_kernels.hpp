@@ -58,12 +58,17 @@ __global__ void __launch_bounds__(kBlock) kq_pre(FDev F, Consts<double> K, Const
         d2q[3] = qsum(q3) - 4.0 * c3;
         d2q[4] = qsum(q4) - 4.0 * c4;
         if (act) {
-            stv<SD>(F.d2q, (size_t)lane * nt + i, sel4(lane, d2q[0], d2q[1], d2q[2], d2q[3]));
+            // this cell's record: lane e writes D2Q(e+1); lane 0 also D2Q5 and
+            // ANU, lane 1 ANURT, lane 2 VOL/DT (VOL when the step is global)
+            rec_slot<SD>(F.drec, n, i, kRecD2q + lane) = (SD)sel4(lane, d2q[0], d2q[1], d2q[2], d2q[3]);
             if (lane == 0) {
-                stv<SD>(F.d2q, (size_t)4 * nt + i, d2q[4]);
-                stv<SD>(F.anu, i, fdiv(anun, anud));
+                rec_slot<SD>(F.drec, n, i, kRecD2q + 4) = (SD)d2q[4];
+                rec_slot<SD>(F.drec, n, i, kRecAnu) = (SD)fdiv(anun, anud);
             } else if (lane == 1) {
-                stv<SD>(F.anurt, i, fdiv(rtn, dmax(rtd, 8.0 * c0 * K.rtol)));
+                rec_slot<SD>(F.drec, n, i, kRecAnurt) = (SD)fdiv(rtn, dmax(rtd, 8.0 * c0 * K.rtol));
+            } else if (lane == 2) {
+                const R vol = (R)F.vol[i];
+                rec_slot<SD>(F.drec, n, i, kRecVbydt) = (SD)(K.local ? fdiv(vol, (R)dt) : vol);
             }
         }
         if (!K.local) block_atomic_min_pos(act ? dt : 100000.0, reinterpret_cast<double *>(F.dtslot));
@@ -122,12 +127,13 @@ __global__ void __launch_bounds__(kBlock) kq_pre(FDev F, Consts<double> K, Const
 #undef TDT_NODE_ACC
     }
     if (actv) {
+        // this node's averages: lane e writes average e, lane 0 also RN
         const R cnt = (R)(j1 - j0);
-        const int m = F.m;
         const Recip<R> rcnt(cnt);
         const R val = rcnt.div(sel4(lane, au, av, at, arho));
-        stv<SD>(F.node, (size_t)lane * m + v, val);
-        if (lane == 0) stv<SD>(F.node, (size_t)4 * m + v, dmax(KR.rtol, rcnt.div(ar)));
+        const int m = F.m;
+        nav_slot<SD>(F.nrec, m, v, lane) = (SD)val;
+        if (lane == 0) nav_slot<SD>(F.nrec, m, v, 4) = (SD)dmax(KR.rtol, rcnt.div(ar));
     }
 }
 
@@ -173,35 +179,29 @@ __global__ void __launch_bounds__(kBlock) kq_stage(FDev F, Consts<double> K, Con
     if (STAGE == 1) {
         // dissipation face terms; a ghost neighbour is a copy of this cell
         const double dti = K.local ? F.dt[i] : __longlong_as_double((long long)*F.dtslot);
-        const R voli = (R)F.vol[i];
-        const R volbydt = fdiv(voli, (R)dti);
-        const R dtj = K.local ? (R)F.dt[jj] : (R)dti;
-        const R volj = (R)F.vol[jj];
+        SD ri[kRecLen], rj[kRecLen];
+        rec_get<SD>(F.drec, n, i, ri);
+        rec_get<SD>(F.drec, n, jj, rj);
+        const R volbydt = rec_vbydt<R>(ri, K.local, dti);
+        const R vbn = rec_vbydt<R>(rj, K.local, dti);
         R a2, a4, b2, b4;
-        jst_coeffs<R>((R)ld<SD>(F.anu, i), (R)ld<SD>(F.anu, jj), volbydt, volj, dtj, a2, a4);
-        jst_coeffs<R>((R)ld<SD>(F.anurt, i), (R)ld<SD>(F.anurt, jj), volbydt, volj, dtj, b2, b4);
+        jst_coeffs_v<R>((R)ri[kRecAnu], (R)rj[kRecAnu], volbydt, vbn, a2, a4);
+        jst_coeffs_v<R>((R)ri[kRecAnurt], (R)rj[kRecAnurt], volbydt, vbn, b2, b4);
 #pragma unroll
         for (int q = 0; q < 5; ++q) {
             const R c2 = q < 4 ? a2 : b2, c4 = q < 4 ? a4 : b4;
             const R dc = (R)(Q[q * nt + jj] - Q[q * nt + i]);
-            const R dq = (R)ld<SD>(F.d2q, (size_t)q * nt + jj) - (R)ld<SD>(F.d2q, (size_t)q * nt + i);
+            const R dq = (R)rj[kRecD2q + q] - (R)ri[kRecD2q + q];
             dcell[q] = (double)(SD)(qsum(c2 * dc) - qsum(c4 * dq));   // as stored for stages 2-4
         }
         // viscous face (viscous.f90:130-217) with nodal averages of this face's nodes
         const int4 nd4 = F.nod4[i];
         const int bnode = sel4(lane, nd4.x, nd4.y, nd4.z, nd4.w);
         const int dnode = sel4(lane, nd4.y, nd4.z, nd4.w, nd4.x);
-        const int m = F.m;
         const R ra = fdiv(pi.c[4], pi.c[0]), rc = fdiv(pj.c[4], pj.c[0]);
-        ViscFace<R> o = viscous_face<R>(
-            sx, sy, (R)F.xcc[i], (R)F.ycc[i], (R)F.xn[bnode], (R)F.yn[bnode], xj, yj,
-            (R)F.xn[dnode], (R)F.yn[dnode], pi.u, pi.v, pi.t, pi.c[0], ra,
-            (R)ld<SD>(F.node, bnode), (R)ld<SD>(F.node, (size_t)m + bnode),
-            (R)ld<SD>(F.node, (size_t)2 * m + bnode), (R)ld<SD>(F.node, (size_t)3 * m + bnode),
-            (R)ld<SD>(F.node, (size_t)4 * m + bnode), pj.u, pj.v, pj.t, pj.c[0], rc,
-            (R)ld<SD>(F.node, dnode), (R)ld<SD>(F.node, (size_t)m + dnode),
-            (R)ld<SD>(F.node, (size_t)2 * m + dnode), (R)ld<SD>(F.node, (size_t)3 * m + dnode),
-            (R)ld<SD>(F.node, (size_t)4 * m + dnode), KR);
+        ViscFace<R> o = viscous_face_rec<R, SD>(F, sx, sy, (R)F.xcc[i], (R)F.ycc[i], pi.u, pi.v,
+                                                pi.t, pi.c[0], ra, bnode, dnode, xj, yj, pj.u,
+                                                pj.v, pj.t, pj.c[0], rc, KR);
         vcell[0] = (double)(SD)qsum2(o.u0, o.u1);
         vcell[1] = (double)(SD)qsum2(o.v0, o.v1);
         vcell[2] = (double)(SD)qsum2(o.t0, o.t1);
