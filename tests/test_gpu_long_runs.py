@@ -110,3 +110,75 @@ def test_duct_20000_fp32_mixed_against_reference(golden):
     print("fp32-mixed vs reference after 20000 iterations:",
           {k: float("%.2e" % v) for k, v in err.items()})
     assert err["p"] <= 1e-5, err
+
+
+needs_ramp = pytest.mark.skipif(
+    not os.path.exists(os.path.join(GOLDEN, "ramp_2049x513_t3.npz")),
+    reason="ramp_2049x513_t3 fixture not generated")
+
+
+def first_nonfinite(l2):
+    bad = ~np.isfinite(l2).all(axis=1)
+    return int(np.argmax(bad)) + 1 if bad.any() else 0
+
+
+@pytest.fixture(scope="module")
+def ramp_t3():
+    from tdt import gridgen
+    mk_mesh, mk_flow = gridgen.WORKLOADS["ramp_2049x513"]
+    return mk_mesh(), mk_flow()
+
+
+@needs_ramp
+def test_ramp_type3_divergence_is_the_reference_s(golden, ramp_t3):
+    """The 2049x513 ramp as specified (type-3 extrapolation at its Mach 2.5
+    inflow) goes non-finite in the reference solver itself: the FP64 path
+    follows the reference's L2 history (1e-10 over the first 1000 iterations,
+    while rounding differences are still small) and goes non-finite at the
+    reference's iteration, within the few iterations over which the last
+    rounding differences blow up."""
+    import tdt
+    g = golden("ramp_2049x513_t3")
+    mesh, flow = ramp_t3
+    assert mesh.neles == g.mesh.neles and np.array_equal(mesh.nc, g.mesh.nc)
+    ref = g["hist_l2"].reshape(-1, 5)
+    bad_ref = first_nonfinite(ref)
+    assert bad_ref > 0, "the reference stayed finite"
+    with tdt.Solver(mesh, flow, "fp64") as s:
+        s.cold_start()
+        l2, _, bad = s.iterate(g.nrun, check=False)
+    k = 1000
+    dev = rel_dev(l2[:k], ref[:k])
+    growth = [rel_dev(l2[a:a + 100], ref[a:a + 100]) for a in range(0, bad_ref - 100, 500)]
+    print(f"reference non-finite at {bad_ref}, FP64 path at {bad}; L2 history over the first "
+          f"{k}: {dev:.2e}; per 500 iterations: {[f'{x:.1e}' for x in growth]}")
+    assert dev <= REL, dev
+    assert bad > 0 and abs(bad - bad_ref) <= 50, (bad, bad_ref)
+
+
+def test_ramp_type3_divergence_follows_perturbation_size(ramp_t3):
+    """Why FP32-mixed goes non-finite earlier than FP64 on this case: the
+    unconstrained inflow column amplifies any perturbation (about a decade
+    every 200 iterations), so the iteration where the state blows up is set by
+    the perturbation's size.  FP64 started from the cold start perturbed by
+    FP32-rounding-sized relative noise (6e-8) blows up about where FP32-mixed
+    does (which adds rounding of that size every iteration), far earlier than
+    the unperturbed FP64 run."""
+    import tdt
+    mesh, flow = ramp_t3
+    bad = {}
+    with tdt.Solver(mesh, flow, "fp64") as s:
+        rinit = s.cold_start()
+        c0 = s.get_state()
+        for eps in (0.0, 1e-9, 6e-8):
+            c = c0 * (1.0 + eps * np.random.default_rng(1).uniform(-1.0, 1.0, c0.shape))
+            s.set_state(c, rinit)
+            bad[eps] = s.iterate(4000, history=False, check=False)[2]
+    with tdt.Solver(mesh, flow, "fp32-mixed") as s:
+        s.cold_start()
+        bad["fp32-mixed"] = s.iterate(4000, history=False, check=False)[2]
+    print("first non-finite iteration:", bad)
+    assert all(v > 0 for v in bad.values()), bad
+    assert bad[6e-8] < bad[1e-9] < bad[0.0]
+    assert bad[6e-8] < 0.6 * bad[0.0]
+    assert abs(bad["fp32-mixed"] - bad[6e-8]) < 0.25 * bad[6e-8], bad
