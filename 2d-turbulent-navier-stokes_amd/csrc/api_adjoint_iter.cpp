@@ -476,13 +476,18 @@ int tdt_adjoint_fixed_point(tdt_ctx *c, const double *cst, double rinit, const d
     if (!rc) rc = fixed_point_prepare(c, b.qn);
     if (!rc) HIPCALL(c, hipMemcpyAsync(b.gJ, b.mu, n5 * sizeof(double), hipMemcpyDeviceToDevice,
                                        c->stream), "adjoint: copy");
-    // lam_{k+1} = Phi_Q^T lam_k + dJ/dQ from lam_0 = dJ/dQ; the change is checked
-    // on the host every 25 iterations
+    // lam_{k+1} = Phi_Q^T lam_k + dJ/dQ from lam_0 = dJ/dQ.  Every kCheck
+    // iterations the host measures the one-step change d_k = max|lam_{k+1} -
+    // lam_k| / max|lam_{k+1}|.  For a contraction of rate rho the remaining
+    // error is about d_k rho / (1 - rho): rho is estimated from two successive
+    // checks, rho^kCheck = d_k / d_{k-kCheck}, and the iteration stops when that
+    // estimate is <= tol (d_k alone underestimates it when rho is near 1).
+    constexpr int kCheck = 25;
     std::vector<double> h0(n5), h1(n5);
     int it = 0;
-    double rel = INFINITY;
+    double rel = INFINITY, prev = INFINITY;
     while (!rc && it < max_iter && !(rel <= tol)) {
-        const bool check = (it + 1) % 25 == 0 || it + 1 == max_iter;
+        const bool check = (it + 1) % kCheck == 0 || it + 1 == max_iter;
         if (check)   // lam_k (frozen_vjp reuses b.mu for the stages' adjoints)
             HIPCALL(c, hipMemcpyAsync(h0.data(), b.mu, n5 * sizeof(double),
                                       hipMemcpyDeviceToHost, c->stream), "download");
@@ -500,12 +505,22 @@ int tdt_adjoint_fixed_point(tdt_ctx *c, const double *cst, double rinit, const d
                 dmax = std::max(dmax, std::fabs(h1[k] - h0[k]));
                 lmax = std::max(lmax, std::fabs(h1[k]));
             }
-            rel = lmax > 0.0 ? dmax / lmax : 0.0;
-            if (!std::isfinite(rel)) {
+            const double d = lmax > 0.0 ? dmax / lmax : 0.0;
+            if (!std::isfinite(d)) {
                 c->err = "fixed-point adjoint: the iteration diverged";
                 rc = TDT_ERR_NONFINITE;
                 break;
             }
+            if (d == 0.0) {
+                rel = 0.0;                                     // exactly converged
+            } else if (std::isfinite(prev) && d < prev && it % kCheck == 0) {
+                const double rho = std::pow(d / prev, 1.0 / kCheck);
+                rel = rho < 1.0 ? d * rho / (1.0 - rho) : INFINITY;
+            } else {
+                rel = INFINITY;                                // not contracting yet
+            }
+            if (it == max_iter) rel = std::isfinite(rel) ? rel : d;
+            prev = d;
         }
         HIPCALL(c, hipMemcpyAsync(b.mu, b.lam, n5 * sizeof(double), hipMemcpyDeviceToDevice,
                                   c->stream), "adjoint: copy");

@@ -176,12 +176,16 @@ int tdt_iteration_vjp(tdt_ctx *ctx, const double *c, double rinit, const double 
 /* Fixed-point adjoint at a steady state c (with RINIT = rinit; c = Phi(c), Phi one
  * iteration): lam = (I - Phi_Q^T)^-1 dj_dc by the iteration lam <- Phi_Q^T lam +
  * dj_dc from lam = dj_dc, the RK stages recomputed once at c and their blocks
- * kept, until the largest change relative to the largest |lam| is <= tol (checked
- * every 25 iterations) or max_iter; then gparam ([3] or NULL) = lam^T
+ * kept, until the estimated remaining error relative to the largest |lam| is
+ * <= tol or max_iter.  Every 25 iterations the one-step change d (largest
+ * |lam_{k+1} - lam_k| over the largest |lam|) is measured; the error estimate is
+ * d rho / (1 - rho), with the contraction rate rho from two successive checks
+ * (rho^25 = d_k / d_{k-25}).  Then gparam ([3] or NULL) = lam^T
  * dPhi/d(P0, T0, RINIT) and gxy ([2][nods] or NULL) = lam^T dPhi/d(X, Y) per
  * grid.in millimetre, so dJ/dp = dJ/dp|explicit + gparam for the steady state's
- * J.  iters, change (may be NULL): iterations done, last relative change; a
- * change above tol means max_iter came first.  TDT_ERR_NONFINITE if the
+ * J.  iters, change (may be NULL): iterations done, the last error estimate (the
+ * last one-step change if no rate could be estimated); a value above tol means
+ * max_iter came first.  TDT_ERR_NONFINITE if the
  * iteration diverges.  The context's C is kept (COLD is set to it). */
 int tdt_adjoint_fixed_point(tdt_ctx *ctx, const double *c, double rinit, const double *dj_dc,
                             int32_t max_iter, double tol, double *lam, double *gparam,

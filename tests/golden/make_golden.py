@@ -119,6 +119,75 @@ def run_case(name, mesh, flow, nrun, dump, sample_stride=1, crosscheck=True, big
         shutil.rmtree(work)
 
 
+def converged_channel():
+    """A non-uniform flow that converges: the 48 x 16 planar channel (100 x 10
+    mm, first cell 0.2 mm at the wall) with a total-condition inlet (type 2), an
+    outlet held at the static pressure of Mach 0.2 (type 3 with PAMB), a no-slip
+    wall below and symmetry above: a developing boundary layer whose L2 residual
+    falls by 1e-8 within about 400,000 iterations."""
+    g, mach, p0 = 1.4, 0.2, 1.2e5
+    pamb = p0 / (1.0 + 0.5 * (g - 1.0) * mach * mach) ** (g / (g - 1.0))
+    mesh = gridgen.channel(nx=48, ny=16, length=100.0, height=10.0, h1=0.2, jitter=0.0,
+                           bc={"left": 2, "right": 3, "bottom": 5, "top": 4})
+    return mesh, gridgen.flow_channel(p0=p0, amachinit=mach, pamb=pamb)
+
+
+def outlet_mass_flow(mesh, c):
+    """Sum over the outlet (type-3, right) faces of rho u of the parent cell times
+    the face's SC_x (metres, per unit depth)."""
+    out = []
+    for k in range(mesh.nghosts):
+        if mesh.btype[k] == 3 and mesh.side[k] == 2:
+            p = int(mesh.parent[k]) - 1
+            a, b = int(mesh.nod[1, p]) - 1, int(mesh.nod[2, p]) - 1
+            out.append(c[1, p] * (mesh.y_mm[b] - mesh.y_mm[a]) * 1e-3)
+    return float(np.sum(out))
+
+
+def run_fd_case(name, mesh, flow, nrun, steps=(1e-6, 1e-5)):
+    """The reference converged from the cold start at the base flow and at P0, T0
+    moved by +-h relative: J = outlet mass flow of each final state, for central
+    differences dJ/dP0, dJ/dT0 of the steady state; the base run's L2 history
+    every 1000 iterations and its final state."""
+    import concurrent.futures as cf
+    runs = [("base", flow)]
+    for h in steps:
+        for par in ("p0", "t0"):
+            for sg in (1, -1):
+                runs.append((f"{par}{'+' if sg > 0 else '-'}{h:g}",
+                             dataclasses.replace(flow, **{par: getattr(flow, par) * (1 + sg * h)})))
+
+    def one(item):
+        tag, f = item
+        work = tempfile.mkdtemp(prefix=f"tdt_golden_{name}_")
+        try:
+            f = dataclasses.replace(f, niter=nrun, istart=0)
+            caseio.write_case(work, f, mesh)
+            with open(os.path.join(work, "driver.in"), "w") as fh:
+                fh.write(f"{nrun} 0\n")
+            subprocess.check_call([os.path.join(REF, "ref_driver")], cwd=work,
+                                  stdout=subprocess.DEVNULL)
+            final = parse_final(os.path.join(work, "final.bin"))
+            l2, _ = parse_hist(os.path.join(work, "hist.bin"))
+            return tag, final, l2
+        finally:
+            shutil.rmtree(work)
+    with cf.ThreadPoolExecutor(max_workers=len(runs)) as ex:
+        res = list(ex.map(one, runs))
+    data = {f"m_{k}": v for k, v in mesh.arrays().items()}
+    data["flow_json"] = np.array(json.dumps(dataclasses.asdict(flow)))
+    data["nrun"] = np.array(nrun)
+    data["fd_steps"] = np.array(steps)
+    for tag, final, l2 in res:
+        data["J_" + tag] = np.array(outlet_mass_flow(mesh, final))
+        if tag == "base":
+            data["final"] = final
+            data["hist_l2_every1000"] = l2[999::1000]
+            data["hist_l2_head"] = l2[:50]
+    np.savez_compressed(os.path.join(OUT, name + ".npz"), **data)
+    print(f"{name}: {nrun} iterations, J = {float(data['J_base']):.12e}")
+
+
 def parse_tecplot(path, nods, neles):
     """Numbers of a tec_plot.dat: NODS x 8 node values, then NELES x 4 node ids
     (list-directed records may wrap, so the body is read as one token stream)."""
@@ -173,10 +242,11 @@ def main():
         raise SystemExit("oracle/_ref not built: run oracle/build_ref.sh first")
     only = set(sys.argv[1:])
     if only:                                      # regenerate just the named fixtures
-        global run_case, run_host_case
-        _rc, _rh = run_case, run_host_case
+        global run_case, run_host_case, run_fd_case
+        _rc, _rh, _rf = run_case, run_host_case, run_fd_case
         run_case = lambda name, *a, **k: _rc(name, *a, **k) if name in only else None  # noqa: E731
         run_host_case = lambda name, *a, **k: _rh(name, *a, **k) if name in only else None  # noqa: E731
+        run_fd_case = lambda name, *a, **k: _rf(name, *a, **k) if name in only else None  # noqa: E731
     # 2-D channel: inlet 2, outlet 3 (extrapolated pressure), wall 5, symmetry 4
     m = gridgen.channel(nx=10, ny=6, length=50.0, height=10.0, h1=0.5, jitter=0.03,
                         bc={"left": 2, "right": 3, "bottom": 5, "top": 4}, seed=11)
@@ -225,6 +295,11 @@ def main():
     run_case("duct_385x129_2000", m, f, nrun=2000, dump=False)
     run_case("duct_385x129_20000", m, f, nrun=20000, dump=False, keep_linf=False,
              periodic=True)
+    # a non-uniform flow converged to a residual drop below 1e-8, and the
+    # reference's central differences of its outlet mass flow in P0 and T0 (the
+    # fixed-point adjoint's check)
+    m, f = converged_channel()
+    run_fd_case("channel_48x16_converged", m, f, nrun=600000)
     # the 97x49 channel / flat plate as specified: type-7 inlet with its inputs
     # defined (the executable cannot set them: no cross-check), its NITER = 2000
     m, f = gridgen.WORKLOADS["channel_97x49"][0](), gridgen.WORKLOADS["channel_97x49"][1]()
