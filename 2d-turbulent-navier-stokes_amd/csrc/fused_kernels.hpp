@@ -31,6 +31,13 @@ __device__ __forceinline__ int cell_block(const FDev &F) {
     return F.remap ? xcd_chunk(blockIdx.x, F.ncb8) : blockIdx.x;
 }
 
+// FP32-mixed stores the frozen dissipation and viscous terms combined, E1 = D1,
+// Eq = Dq + VIS(q-1) (the stage updates subtract them together), five values
+// per cell instead of nine; FP64 keeps D and VIS apart, as the reference
+// subtracts them one after the other.
+template <class SD>
+constexpr bool kCombinedDV = sizeof(SD) == 4;
+
 template <class T>
 __device__ __forceinline__ double ld(const void *p, size_t i) {
     return (double)static_cast<const T *>(p)[i];
@@ -550,10 +557,13 @@ __global__ void __launch_bounds__(kBlock) k_dv(FDev F, Consts<double> K, Consts<
     // ---- dissipation (solver_state.f90:183-223); ghost neighbour = parent copy
     double ci[5];
     for (int q = 0; q < 5; ++q) ci[q] = Q[q * nt + i];
+    R dsum[5];
     {
         const int nbr[4] = {raw[0] & kIdMask, raw[1] & kIdMask, raw[2] & kIdMask, raw[3] & kIdMask};
-        R dsum[5];
         dissipation_rec<R, SD>(F, i, nbr, ci, K.local, dti, dsum);
+        // stored now in either form (FP32-mixed overwrites it with D + VIS
+        // below): the store keeps the viscous part's loads after this point,
+        // which holds the register count down
         for (int q = 0; q < 5; ++q) stv<SD>(F.dd, (size_t)q * n + i, dsum[q]);
     }
     // ---- viscous fluxes (viscous.f90:62-70, 130-217)
@@ -602,10 +612,15 @@ __global__ void __launch_bounds__(kBlock) k_dv(FDev F, Consts<double> K, Consts<
         vt = vt + o.t0 + o.t1;
         vr = vr + o.r0 + o.r1;
     }
-    stv<SD>(F.vis, i, vu);
-    stv<SD>(F.vis, (size_t)n + i, vv);
-    stv<SD>(F.vis, (size_t)2 * n + i, vt);
-    stv<SD>(F.vis, (size_t)3 * n + i, vr);
+    if constexpr (kCombinedDV<SD>) {
+        const R e[5] = {dsum[0], dsum[1] + vu, dsum[2] + vv, dsum[3] + vt, dsum[4] + vr};
+        for (int q = 0; q < 5; ++q) stv<SD>(F.dd, (size_t)q * n + i, e[q]);
+    } else {
+        stv<SD>(F.vis, i, vu);
+        stv<SD>(F.vis, (size_t)n + i, vv);
+        stv<SD>(F.vis, (size_t)2 * n + i, vt);
+        stv<SD>(F.vis, (size_t)3 * n + i, vr);
+    }
 }
 
 // ---------------------------------------------------------------------------------
@@ -702,26 +717,30 @@ __global__ void __launch_bounds__(kBlock) k_rk(FDev F, Consts<double> K, Consts<
         } else {
             dti = F.dt[i];
         }
-        double dd[5], vs[4];
-        // this cell's four faces' viscous terms (requested before anything waits)
+        double dd[5], vs[4] = {0.0, 0.0, 0.0, 0.0};
+        R dsum[5];
         if (kInlineDV) {
             // dissipation faces; a ghost neighbour is a copy of this cell
             double ci[5];
 #pragma unroll
             for (int q = 0; q < 5; ++q) ci[q] = Q[q * nt + i];
             const int nbr[4] = {nb4.x, nb4.y, nb4.z, nb4.w};
-            R dsum[5];
             dissipation_rec<R, SD>(F, i, nbr, ci, K.local, dti, dsum);
+            // stored now in either form (FP32-mixed overwrites it with D + VIS
+            // after the faces): the store keeps the face loop's loads after it
 #pragma unroll
             for (int q = 0; q < 5; ++q) {
                 dd[q] = (double)(SD)dsum[q];
                 stv<SD>(F.dd, (size_t)q * n + i, dd[q]);
             }
         } else {
+            // D and VIS frozen at stage 1 (FP32-mixed: combined in D)
 #pragma unroll
             for (int q = 0; q < 5; ++q) dd[q] = ld<SD>(F.dd, (size_t)q * n + i);
+            if constexpr (!kCombinedDV<SD>) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) vs[q] = ld<SD>(F.vis, (size_t)q * n + i);
+                for (int q = 0; q < 4; ++q) vs[q] = ld<SD>(F.vis, (size_t)q * n + i);
+            }
         }
 
         // FP32: every face's neighbour state is loaded up front (a ghost face
@@ -826,10 +845,19 @@ __global__ void __launch_bounds__(kBlock) k_rk(FDev F, Consts<double> K, Consts<
             }
         }
         if (kInlineDV) {
-            vs[0] = (double)(SD)vu; vs[1] = (double)(SD)vv;
-            vs[2] = (double)(SD)vt; vs[3] = (double)(SD)vr;
+            if constexpr (kCombinedDV<SD>) {
+                const R e[5] = {dsum[0], dsum[1] + vu, dsum[2] + vv, dsum[3] + vt, dsum[4] + vr};
 #pragma unroll
-            for (int q = 0; q < 4; ++q) stv<SD>(F.vis, (size_t)q * n + i, vs[q]);
+                for (int q = 0; q < 5; ++q) {
+                    dd[q] = (double)(SD)e[q];
+                    stv<SD>(F.dd, (size_t)q * n + i, dd[q]);
+                }
+            } else {
+                vs[0] = (double)(SD)vu; vs[1] = (double)(SD)vv;
+                vs[2] = (double)(SD)vt; vs[3] = (double)(SD)vr;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) stv<SD>(F.vis, (size_t)q * n + i, vs[q]);
+            }
         }
         R s3;
         const R s5 = source_terms_d<R>(sd, pi.c[0], pi.c[4], pi.t, pi.p, pi.v,

@@ -281,28 +281,38 @@ __global__ void __launch_bounds__(kBlock) kq_stage(FDev F, Consts<double> K, Con
     const int e = lane;
     const double coldE = Q[e * nt + i];
     double dE, visE;
+    constexpr bool kE = kCombinedDV<SD>;   // FP32-mixed: E = D + VIS stored in D
     if (STAGE == 1) {
         dE = sel4(e, dcell[0], dcell[1], dcell[2], dcell[3]);
         visE = sel4(e, 0.0, vcell[0], vcell[1], vcell[2]);
+        if (kE) {
+            dE = (double)(SD)(dE + visE);
+            visE = 0.0;
+        }
     } else {
         dE = ld<SD>(F.dd, (size_t)e * n + i);
-        visE = e == 0 ? 0.0 : ld<SD>(F.vis, (size_t)(e - 1) * n + i);
+        visE = e == 0 || kE ? 0.0 : ld<SD>(F.vis, (size_t)(e - 1) * n + i);
     }
     const double rE = sel4(e, rr[0], rr[1], rr[2], rr[3]);
     double cE = e == 0 ? coldE - a * (rE - dE) : coldE - a * (rE - visE - dE);
-    double cold4 = 0.0, c4 = 0.0;
+    double cold4 = 0.0, c4 = 0.0, e4 = 0.0;
     if (lane == 0) {
         cold4 = Q[4 * nt + i];
-        const double d4 = STAGE == 1 ? dcell[4] : ld<SD>(F.dd, (size_t)4 * n + i);
-        const double v4 = STAGE == 1 ? vcell[3] : ld<SD>(F.vis, (size_t)3 * n + i);
+        double d4 = STAGE == 1 ? dcell[4] : ld<SD>(F.dd, (size_t)4 * n + i);
+        double v4 = STAGE == 1 ? vcell[3] : (kE ? 0.0 : ld<SD>(F.vis, (size_t)3 * n + i));
+        if (kE && STAGE == 1) {
+            d4 = (double)(SD)(d4 + v4);
+            v4 = 0.0;
+        }
+        e4 = d4;
         c4 = cold4 - a * kCflTurb * (rr[4] - v4 - d4);
         c4 = cE * dmax(K.rtol, fdiv(c4, cE));
     }
     if (act) {
         if (STAGE == 1) {
             stv<SD>(F.dd, (size_t)e * n + i, dE);
-            stv<SD>(F.vis, (size_t)e * n + i, sel4(e, vcell[0], vcell[1], vcell[2], vcell[3]));
-            if (lane == 0) stv<SD>(F.dd, (size_t)4 * n + i, dcell[4]);
+            if (!kE) stv<SD>(F.vis, (size_t)e * n + i, sel4(e, vcell[0], vcell[1], vcell[2], vcell[3]));
+            if (lane == 0) stv<SD>(F.dd, (size_t)4 * n + i, e4);
         }
         if (STAGE < 4) {
             void *qo = F.st[(STAGE - 1) & 1];

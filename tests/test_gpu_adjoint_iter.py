@@ -567,3 +567,66 @@ def test_fixed_point_adjoint_at_a_steady_state():
                   - steady_outlet_mass_flow(mesh, flow, **args[1])) / (2 * h)
             check(fd, gp[k], abs(fd), 1e-6, name)
         assert abs(gp[2]) * rinit < 1e-12 * abs(gp[0]) * flow.p0
+
+
+CONVERGED = "channel_48x16_converged"
+
+
+@pytest.mark.skipif(not __import__("os").path.exists(__import__("os").path.join(
+    REPO, "tests", "golden", CONVERGED + ".npz")), reason=CONVERGED + " fixture not generated")
+def test_fixed_point_adjoint_at_a_converged_nonuniform_flow(golden):
+    """The fixed-point adjoint at a developed, non-uniform steady state: the
+    48 x 16 channel (total-condition inlet, outlet at a fixed static pressure,
+    no-slip wall, symmetry) converged from the cold start to an L2 residual drop
+    below 1e-8 (a boundary layer grows along the wall; the state is nowhere
+    uniform).  J = outlet mass flow.  The steady state's dJ/dP0 and dJ/dT0 from
+    lam^T dPhi/dp (the cold start's RINIT = f(P0, T0) chained in) against
+    central differences (relative step 1e-6) of converged runs of this library
+    and of the reference solver (tests/golden: its converged runs at P0, T0
+    moved by +-1e-6), within 1e-4.  The converged state's J equals the
+    reference's to 1e-9."""
+    import dataclasses as dc
+    import tdt
+    from tdt import objective
+    g = golden(CONVERGED)
+    mesh, flow, n = g.mesh, g.flow, g.nrun
+    out = objective.outlet_faces(mesh)
+    sx, _, _ = objective._scx(mesh, 0, out[:, 1], out[:, 2])
+
+    def J(c):
+        return float(np.sum(c[1, out[:, 0]] * sx))
+
+    with tdt.Solver(mesh, flow) as s:
+        rinit = s.cold_start()
+        l2, _, bad = s.iterate(n)
+        assert bad == 0
+        drop = float(np.max(l2[-1] / l2[:50].max(axis=0)))
+        c = s.get_state()
+        j = J(c)
+        assert drop <= 1e-8, drop
+        assert abs(j - float(g["J_base"])) <= 1e-9 * abs(float(g["J_base"])), (j, g["J_base"])
+        dj = np.zeros_like(c)
+        np.add.at(dj[1], out[:, 0], sx)
+        _, gp, iters, est = s.adjoint_fixed_point(c, rinit, dj, max_iter=1500000, tol=1e-8)
+        assert est <= 1e-8, (iters, est)
+        # RINIT = AMUINIT / RHOINIT of the cold start: proportional to T0^1.6 / P0
+        adj = {"p0": gp[0] - gp[2] * rinit / flow.p0,
+               "t0": gp[1] + gp[2] * 1.6 * rinit / flow.t0}
+        fd = {}
+        for par in ("p0", "t0"):
+            base = getattr(flow, par)
+            jj = []
+            for sg in (1.0, -1.0):
+                s.set_flow(dc.replace(flow, **{par: base * (1.0 + sg * 1e-6)}))
+                s.cold_start()
+                s.iterate(n, history=False)
+                jj.append(J(s.get_state()))
+            s.set_flow(flow)
+            fd[par] = (jj[0] - jj[1]) / (2e-6 * base)
+    ref = {par: (float(g[f"J_{par}+1e-06"]) - float(g[f"J_{par}-1e-06"]))
+           / (2e-6 * getattr(flow, par)) for par in ("p0", "t0")}
+    print(f"converged to an L2 drop of {drop:.2e}; adjoint {iters} iterations (error estimate "
+          f"{est:.1e}); dJ/dp adjoint {adj}, differences {fd}, reference differences {ref}")
+    for par in ("p0", "t0"):
+        check(fd[par], adj[par], abs(fd[par]), 1e-4, par + " vs differences")
+        check(ref[par], adj[par], abs(ref[par]), 1e-4, par + " vs reference differences")
