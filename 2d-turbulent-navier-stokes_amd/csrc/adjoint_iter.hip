@@ -71,23 +71,69 @@ struct OpGeoSlots {
     const Dev *D;
     int i;
     int nb[4];
-    int node[kNodeSlots];
-    R xs[kNodeSlots], ys[kNodeSlots];
+    int d0;                 // first direction: direction d0 + k seeds slot (d0 + k) / 2,
+                            // its x if even, its y if odd
+    int vfirst[kDualN];     // the node whose lookups by id carry direction k: the
+                            // seeded slot's node if that slot is its first, else -1
     R xcc[5], ycc[5], volc[5], dlc[5];     // the cell, then across faces 1..4
     R scv[8];
 
-    __device__ int slot(int v) const {
-        for (int s = 0; s < kNodeSlots; ++s)
-            if (node[s] == v) return s;
-        return 0;   // not reached for the nodes the operators read
+    __device__ int node_of(int s) const {
+        const int *nod = reinterpret_cast<const int *>(D->nod4);
+        if (s < 4) return nod[4 * i + s];
+        const int j = nb[(s - 4) >> 2];
+        return j < D->n ? nod[4 * j + ((s - 4) & 3)] : -1;
+    }
+    // the coordinates of slot s, seeded (read from the node arrays on demand:
+    // kept in arrays, their lookups by id would index them dynamically)
+    __device__ R xs(int s) const {
+        const int v = node_of(s);
+        R r(v >= 0 ? D->xn[v] : 0.0);
+        for (int k = 0; k < kDualN; ++k)
+            if (d0 + k == 2 * s) r.d[k] = 1.0;
+        return r;
+    }
+    __device__ R ys(int s) const {
+        const int v = node_of(s);
+        R r(v >= 0 ? D->yn[v] : 0.0);
+        for (int k = 0; k < kDualN; ++k)
+            if (d0 + k == 2 * s + 1) r.d[k] = 1.0;
+        return r;
+    }
+    __device__ void seed(int first_dir) {
+        d0 = first_dir;
+        for (int k = 0; k < kDualN; ++k) {
+            vfirst[k] = -1;
+            const int sl = (d0 + k) >> 1;
+            if (d0 + k >= 2 * kNodeSlots) continue;
+            const int v = node_of(sl);
+            int first = sl;
+            for (int t = 0; t < sl; ++t)
+                if (node_of(t) == v) {
+                    first = t;
+                    break;
+                }
+            if (first == sl) vfirst[k] = v;
+        }
     }
     __device__ int face_of(int id) const {
         for (int f = 0; f < 4; ++f)
             if (nb[f] == id) return f;
         return -1;
     }
-    __device__ R x(int v) const { return xs[slot(v)]; }
-    __device__ R y(int v) const { return ys[slot(v)]; }
+    // a node's coordinates by id: those of the first slot holding it
+    __device__ R x(int v) const {
+        R r(D->xn[v]);
+        for (int k = 0; k < kDualN; ++k)
+            if (vfirst[k] == v && ((d0 + k) & 1) == 0) r.d[k] = 1.0;
+        return r;
+    }
+    __device__ R y(int v) const {
+        R r(D->yn[v]);
+        for (int k = 0; k < kDualN; ++k)
+            if (vfirst[k] == v && ((d0 + k) & 1) == 1) r.d[k] = 1.0;
+        return r;
+    }
     __device__ void centroid(int id, R &cx, R &cy) const {
         if (id == i) { cx = xcc[0]; cy = ycc[0]; return; }
         const int f = face_of(id);
@@ -97,8 +143,8 @@ struct OpGeoSlots {
         const int fj = face_of(par);
         const int b = 4 + 4 * fj;
         const int a = b + sd - 1, c = b + (sd & 3);
-        const R xf = R(0.5) * (xs[a] + xs[c]);
-        const R yf = R(0.5) * (ys[a] + ys[c]);
+        const R xf = R(0.5) * (xs(a) + xs(c));
+        const R yf = R(0.5) * (ys(a) + ys(c));
         cx = R(2.0) * xf - xcc[1 + fj];
         cy = R(2.0) * yf - ycc[1 + fj];
     }
@@ -151,21 +197,21 @@ __device__ void quad_geo(const R (&x)[4], const R (&y)[4], int iaxis, R &xc, R &
 template <class R>
 __device__ void slots_finish(OpGeoSlots<R> &G, int iaxis) {
     {
-        const R x[4] = {G.xs[0], G.xs[1], G.xs[2], G.xs[3]};
-        const R y[4] = {G.ys[0], G.ys[1], G.ys[2], G.ys[3]};
+        const R x[4] = {G.xs(0), G.xs(1), G.xs(2), G.xs(3)};
+        const R y[4] = {G.ys(0), G.ys(1), G.ys(2), G.ys(3)};
         quad_geo(x, y, iaxis, G.xcc[0], G.ycc[0], G.dlc[0], G.volc[0], G.scv);
     }
     for (int f = 0; f < 4; ++f) {
         if (G.nb[f] < G.D->n) {
             const int b = 4 + 4 * f;
-            const R x[4] = {G.xs[b], G.xs[b + 1], G.xs[b + 2], G.xs[b + 3]};
-            const R y[4] = {G.ys[b], G.ys[b + 1], G.ys[b + 2], G.ys[b + 3]};
+            const R x[4] = {G.xs(b), G.xs(b + 1), G.xs(b + 2), G.xs(b + 3)};
+            const R y[4] = {G.ys(b), G.ys(b + 1), G.ys(b + 2), G.ys(b + 3)};
             quad_geo(x, y, iaxis, G.xcc[1 + f], G.ycc[1 + f], G.dlc[1 + f], G.volc[1 + f],
                      (R *)nullptr);
         } else {   // k_geometry_ghosts: mirrored centroid, VOL(NG) = VOL(NP)
             const int a = f, c = (f + 1) & 3;
-            const R xf = R(0.5) * (G.xs[a] + G.xs[c]);
-            const R yf = R(0.5) * (G.ys[a] + G.ys[c]);
+            const R xf = R(0.5) * (G.xs(a) + G.xs(c));
+            const R yf = R(0.5) * (G.ys(a) + G.ys(c));
             G.xcc[1 + f] = R(2.0) * xf - G.xcc[0];
             G.ycc[1 + f] = R(2.0) * yf - G.ycc[0];
             G.volc[1 + f] = G.volc[0];
@@ -234,113 +280,135 @@ __device__ void prim_of(const Dev &D, int id, const QL &ql, const GEO &G, const 
     ghost_state<R>(s, ty, pp, uo, vo, to, po, G.yc(id), K);
 }
 
+// k_timestep's C, P and DT of cell x and DISSIPATION's sensors ANU, ANURT and
+// second differences D2Q there (a ghost neighbour of x counts as a copy of x)
+template <class R, class QL, class GEO>
+__device__ __forceinline__ void cell_sensors(const Dev &D, int x, const QL &ql, const GEO &G,
+                                             const Consts<R> &K, R (&cx)[5], R &dt, R &anu,
+                                             R &anurt, R (&d2q)[5]) {
+    const int n = D.n;
+    R px;
+    ql(x, cx);
+    tstep<R>(cx, G.dl(x), K, px, dt);
+    const int4 y4 = D.nc4[x];
+    const int ys[4] = {y4.x, y4.y, y4.z, y4.w};
+    R an, ad, bn, bd, sum[5];
+    for (int k = 0; k < 4; ++k) {
+        R cy[5], py;
+        if (ys[k] < n) {
+            ql(ys[k], cy);
+            py = tstep_p<R>(cy, K);
+        } else {
+            for (int m = 0; m < 5; ++m) cy[m] = cx[m];
+            py = px;
+        }
+        const R tn = fabs(py - px), td = fabs(py + px);
+        const R un = fabs(cy[4] - cx[4]), ud = fabs(cy[4] + cx[4]);
+        if (k == 0) {
+            an = tn; ad = td; bn = un; bd = ud;
+            for (int m = 0; m < 5; ++m) sum[m] = cy[m];
+        } else {
+            an = an + tn; ad = ad + td; bn = bn + un; bd = bd + ud;
+            for (int m = 0; m < 5; ++m) sum[m] = sum[m] + cy[m];
+        }
+    }
+    anu = an / ad;
+    anurt = bn / dmax(bd, R(8.0) * cx[0] * K.rtol);
+    for (int m = 0; m < 5; ++m) d2q[m] = sum[m] - R(4.0) * cx[m];
+}
+
+// k_nodeavg's UN, VN, TN, RHON, RN at node v (the cells and ghosts around it,
+// in the node list's order)
+template <class R, class QL, class GEO>
+__device__ __forceinline__ void node_avg(const Dev &D, int v, const QL &ql, const GEO &G,
+                                         const Consts<R> &K, R (&a)[5]) {
+    const int4 ov = D.novr[v];
+    const int j0 = D.nptr[v], j1 = D.nptr[v + 1];
+    R au = R(0), av = R(0), at = R(0), arho = R(0);
+    R ar = ov.w ? K.rtol_rtred : R(0);
+    for (int j = j0; j < j1; ++j) {
+        Prim<R> s;
+        prim_of<R>(D, D.nlist[j], ql, G, K, s);
+        const R rho = s.c[0], rr = s.c[4] / rho;
+        if (j >= ov.x) au = au + s.u;
+        if (j >= ov.y) av = av + s.v;
+        at = at + s.t;
+        arho = arho + rho;
+        if (j >= ov.z) ar = ar + rr;
+    }
+    const R cnt = R((double)(j1 - j0));
+    a[0] = au / cnt;
+    a[1] = av / cnt;
+    a[2] = at / cnt;
+    a[3] = arho / cnt;
+    a[4] = dmax(K.rtol, ar / cnt);
+}
+
 template <class R, class QL, class GEO>
 __device__ void cell_ops(const Dev &D, int i, const QL &ql, const GEO &G, const Consts<R> &K,
                          OpOut<R> &o) {
     const int n = D.n;
     const int4 nb4 = D.nc4[i];
     const int nb[4] = {nb4.x, nb4.y, nb4.z, nb4.w};
-    // --- DISSIPATION with the timestep's P and DT: per x in {i, face neighbours}
-    // C, P, DT, ANU, ANURT, D2Q (a ghost: its parent's, ANURT 0)
-    R cx[5][5], px[5], dtx[5], anu[5], anurt[5], d2q[5][5], volx[5];
-    for (int s = 0; s < 5; ++s) {
-        const int x = s == 0 ? i : nb[s - 1];
-        if (s > 0 && x >= n) continue;
-        ql(x, cx[s]);
-        tstep<R>(cx[s], G.dl(x), K, px[s], dtx[s]);
-        volx[s] = G.vol(x);
-        const int4 y4 = D.nc4[x];
-        const int ys[4] = {y4.x, y4.y, y4.z, y4.w};
-        R an, ad, bn, bd, sum[5];
-        for (int k = 0; k < 4; ++k) {
-            R cy[5], py;
-            if (ys[k] < n) {
-                ql(ys[k], cy);
-                py = tstep_p<R>(cy, K);
-            } else {
-                for (int m = 0; m < 5; ++m) cy[m] = cx[s][m];
-                py = px[s];
+    // --- DISSIPATION with the timestep's P and DT: the cell's C, DT, sensors and
+    // D2Q, then each face neighbour's (a ghost: the cell's, ANURT 0), whose face
+    // terms are added as soon as they are known (each D2/D4 sum still runs over
+    // the faces in order), so only one neighbour's values are live at a time
+    R c0[5], d2q0[5], dt0, anu0, anurt0;
+    cell_sensors<R>(D, i, ql, G, K, c0, dt0, anu0, anurt0, d2q0);
+    const R vol0 = G.vol(i);
+    const R volbydt = vol0 / dt0;
+    R d2[5], d4[5];
+    for (int k = 0; k < 4; ++k) {
+        const int x = nb[k];
+        R cx[5], d2qx[5], dtx, anux, anurtx, volx;
+        if (x < n) {
+            cell_sensors<R>(D, x, ql, G, K, cx, dtx, anux, anurtx, d2qx);
+            volx = G.vol(x);
+        } else {
+            for (int m = 0; m < 5; ++m) {
+                cx[m] = c0[m];
+                d2qx[m] = d2q0[m];
             }
-            const R tn = fabs(py - px[s]), td = fabs(py + px[s]);
-            const R un = fabs(cy[4] - cx[s][4]), ud = fabs(cy[4] + cx[s][4]);
-            if (k == 0) {
-                an = tn; ad = td; bn = un; bd = ud;
-                for (int m = 0; m < 5; ++m) sum[m] = cy[m];
-            } else {
-                an = an + tn; ad = ad + td; bn = bn + un; bd = bd + ud;
-                for (int m = 0; m < 5; ++m) sum[m] = sum[m] + cy[m];
-            }
+            dtx = dt0;
+            anux = anu0;
+            anurtx = R(0);
+            volx = vol0;
         }
-        anu[s] = an / ad;
-        anurt[s] = bn / dmax(bd, R(8.0) * cx[s][0] * K.rtol);
-        for (int m = 0; m < 5; ++m) d2q[s][m] = sum[m] - R(4.0) * cx[s][m];
-    }
-    for (int s = 1; s < 5; ++s) {
-        if (nb[s - 1] < n) continue;
+        R a2, a4, b2, b4;
+        jst_coeffs<R>(anu0, anux, volbydt, volx, dtx, a2, a4);
+        jst_coeffs<R>(anurt0, anurtx, volbydt, volx, dtx, b2, b4);
         for (int m = 0; m < 5; ++m) {
-            cx[s][m] = cx[0][m];
-            d2q[s][m] = d2q[0][m];
+            const R t2 = (m < 4 ? a2 : b2) * (cx[m] - c0[m]);
+            const R t4 = (m < 4 ? a4 : b4) * (d2qx[m] - d2q0[m]);
+            d2[m] = k == 0 ? t2 : d2[m] + t2;
+            d4[m] = k == 0 ? t4 : d4[m] + t4;
         }
-        px[s] = px[0];
-        dtx[s] = dtx[0];
-        anu[s] = anu[0];
-        anurt[s] = R(0);
-        volx[s] = volx[0];
     }
-    const R volbydt = volx[0] / dtx[0];
-    for (int m = 0; m < 5; ++m) {
-        const R *sens = m < 4 ? anu : anurt;
-        R d2, d4;
-        for (int k = 0; k < 4; ++k) {
-            R d2n, d4n;
-            jst_coeffs<R>(sens[0], sens[1 + k], volbydt, volx[1 + k], dtx[1 + k], d2n, d4n);
-            if (k == 0) {
-                d2 = d2n * (cx[1 + k][m] - cx[0][m]);
-                d4 = d4n * (d2q[1 + k][m] - d2q[0][m]);
-            } else {
-                d2 = d2 + d2n * (cx[1 + k][m] - cx[0][m]);
-                d4 = d4 + d4n * (d2q[1 + k][m] - d2q[0][m]);
-            }
-        }
-        o.d[m] = d2 - d4;
-    }
-    o.dt = dtx[0];
-    o.vol = volx[0];
-    // --- COMPUTE_VISCOUS at stage 1: node averages at the cell's nodes, then faces
+    for (int m = 0; m < 5; ++m) o.d[m] = d2[m] - d4[m];
+    o.dt = dt0;
+    o.vol = vol0;
+    // --- COMPUTE_VISCOUS at stage 1: the faces in order, each between nodes f
+    // and f + 1, whose averages are formed as the faces reach them (node 1's is
+    // kept for face 4)
     const int4 nd4 = D.nod4[i];
     const int nd[4] = {nd4.x, nd4.y, nd4.z, nd4.w};
-    R na[4][5];
-    for (int k = 0; k < 4; ++k) {
-        const int v = nd[k];
-        const int4 ov = D.novr[v];
-        const int j0 = D.nptr[v], j1 = D.nptr[v + 1];
-        R au = R(0), av = R(0), at = R(0), arho = R(0);
-        R ar = ov.w ? K.rtol_rtred : R(0);
-        for (int j = j0; j < j1; ++j) {
-            Prim<R> s;
-            prim_of<R>(D, D.nlist[j], ql, G, K, s);
-            const R rho = s.c[0], rr = s.c[4] / rho;
-            if (j >= ov.x) au = au + s.u;
-            if (j >= ov.y) av = av + s.v;
-            at = at + s.t;
-            arho = arho + rho;
-            if (j >= ov.z) ar = ar + rr;
-        }
-        const R cnt = R((double)(j1 - j0));
-        na[k][0] = au / cnt;
-        na[k][1] = av / cnt;
-        na[k][2] = at / cnt;
-        na[k][3] = arho / cnt;
-        na[k][4] = dmax(K.rtol, ar / cnt);
-    }
     Prim<R> pi;
-    prim_from_q<R>(pi, cx[0][0], cx[0][1], cx[0][2], cx[0][3], cx[0][4], K);
+    prim_from_q<R>(pi, c0[0], c0[1], c0[2], c0[3], c0[4], K);
     const R xa = G.xc(i), ya = G.yc(i);
     const R rhoa = pi.c[0], ra = pi.c[4] / rhoa;
+    R n1[5], B[5];
+    node_avg<R>(D, nd[0], ql, G, K, n1);
+    for (int m = 0; m < 5; ++m) B[m] = n1[m];
     R vu = R(0), vv = R(0), vt = R(0), vr = R(0);
     for (int f = 0; f < 4; ++f) {
         const int b = nd[f], d = nd[(f + 1) & 3], c = nb[f];
-        const R *B = na[f], *Dn = na[(f + 1) & 3];
+        R Dn[5];
+        if (f < 3) {
+            node_avg<R>(D, d, ql, G, K, Dn);
+        } else {
+            for (int m = 0; m < 5; ++m) Dn[m] = n1[m];
+        }
         Prim<R> pc;
         prim_of<R>(D, c, ql, G, K, pc);
         const R rhoc = pc.c[0];
@@ -352,6 +420,7 @@ __device__ void cell_ops(const Dev &D, int i, const QL &ql, const GEO &G, const 
         vv = vv + w.v0 + w.v1;
         vt = vt + w.t0 + w.t1;
         vr = vr + w.r0 + w.r1;
+        for (int m = 0; m < 5; ++m) B[m] = Dn[m];
     }
     o.vis[0] = vu; o.vis[1] = vv; o.vis[2] = vt; o.vis[3] = vr;
 }
@@ -411,90 +480,84 @@ constexpr int kMemberGroups = (5 + kDualN - 1) / kDualN;             // per memb
 constexpr int kParGroups = (3 + kDualN - 1) / kDualN;                // P0, T0, RTOL
 constexpr int kNodeGroups = (2 * kNodeSlots + kDualN - 1) / kDualN;  // slot x, y
 
-// One dual evaluation per thread, blockIdx.y = e: for cell i, member m = e /
-// kMemberGroups (m < its member count) with components c0.. of that member seeded
-// (JE[mptr[i] + m] = d(E_i, DT_i)/dQ_member [row][c]); then the parameter groups
-// (JEp[i] = dE_i/d(P0, T0, RTOL) [row][p]); then the node groups (JEx[i] =
-// d(E_i, DT_i, VOL_i)/d(slot x, y)).  kmax = the largest member count.
-__global__ void __launch_bounds__(64) k_op_jac(Dev D, Consts<double> K, OpSeeds ps,
-                                               const double *qs, const int *mptr,
-                                               const int *mlist, int kmax, double *JE,
-                                               double *JEp, double *JEx) {
+// One dual evaluation per thread, in three kernels (each branch gets the
+// registers it needs alone).  k_op_jac, blockIdx.y = e: for cell i, member m = e
+// / kMemberGroups (m < its member count) with components c0.. of that member
+// seeded: JE[mptr[i] + m] = d(E_i, DT_i)/dQ_member [row][c].  k_op_jac_par,
+// blockIdx.y = e: parameter group e, JEp[i] = dE_i/d(P0, T0, RTOL) [row][p].
+// k_op_jac_xy, blockIdx.y = e: node group e, JEx[i] = d(E_i, DT_i, VOL_i)/d(slot
+// x, y).
+__global__ void __launch_bounds__(64, 2) k_op_jac(Dev D, Consts<double> K, const double *qs,
+                                               const int *mptr, const int *mlist, double *JE) {
     const int i = blockIdx.x * 64 + threadIdx.x;
-    int e = blockIdx.y;
+    const int e = blockIdx.y;
     if (i >= D.n) return;
     const int n = D.n;
     const Consts<Dual<kDualN>> KD = lift<kDualN>(K);
-    if (e < kmax * kMemberGroups) {
-        const int ent = mptr[i] + e / kMemberGroups;
-        const int c0 = (e % kMemberGroups) * kDualN;
-        if (ent >= mptr[i + 1]) return;
-        OpGeoStored<Dual<kDualN>> G{&D, i};
-        OpOut<Dual<kDualN>> o;
-        cell_ops<Dual<kDualN>>(D, i, SeedLoader{qs, n, mlist[ent], c0}, G, KD, o);
-        double *B = JE + (size_t)ent * kOpBlock;
-        for (int k = 0; k < kDualN && c0 + k < 5; ++k) {
-            const int c = c0 + k;
-            B[c] = o.d[0].d[k];
-            for (int m = 1; m < 5; ++m) B[m * 5 + c] = o.d[m].d[k] + o.vis[m - 1].d[k];
-            B[25 + c] = o.dt.d[k];
-        }
-        return;
+    const int ent = mptr[i] + e / kMemberGroups;
+    const int c0 = (e % kMemberGroups) * kDualN;
+    if (ent >= mptr[i + 1]) return;
+    OpGeoStored<Dual<kDualN>> G{&D, i};
+    OpOut<Dual<kDualN>> o;
+    cell_ops<Dual<kDualN>>(D, i, SeedLoader{qs, n, mlist[ent], c0}, G, KD, o);
+    double *B = JE + (size_t)ent * kOpBlock;
+    for (int k = 0; k < kDualN && c0 + k < 5; ++k) {
+        const int c = c0 + k;
+        B[c] = o.d[0].d[k];
+        for (int m = 1; m < 5; ++m) B[m * 5 + c] = o.d[m].d[k] + o.vis[m - 1].d[k];
+        B[25 + c] = o.dt.d[k];
     }
-    e -= kmax * kMemberGroups;
-    if (e < kParGroups) {   // parameter p = e * kDualN + k in direction k
-        Consts<Dual<kDualN>> KP = KD;
-        for (int k = 0; k < kDualN; ++k) {
-            const int p = e * kDualN + k;
-            if (p == 0) KP.s0.d[k] = ps.ds0_dp0;
-            if (p == 1) {
-                KP.s0.d[k] = ps.ds0_dt0;
-                KP.t0.d[k] = 1.0;
-                KP.tinf.d[k] = ps.dtinf_dt0;
-                KP.bl_u.d[k] = ps.dblu_dt0;
-            }
-            if (p == 2) {
-                KP.rtol.d[k] = 1.0;
-                KP.rtol_rtred.d[k] = ps.rtred;
-            }
+}
+
+__global__ void __launch_bounds__(64) k_op_jac_par(Dev D, Consts<double> K, OpSeeds ps,
+                                                   const double *qs, double *JEp) {
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int e = blockIdx.y;
+    if (i >= D.n) return;
+    const int n = D.n;
+    const Consts<Dual<kDualN>> KD = lift<kDualN>(K);
+    // parameter p = e * kDualN + k in direction k
+    Consts<Dual<kDualN>> KP = KD;
+    for (int k = 0; k < kDualN; ++k) {
+        const int p = e * kDualN + k;
+        if (p == 0) KP.s0.d[k] = ps.ds0_dp0;
+        if (p == 1) {
+            KP.s0.d[k] = ps.ds0_dt0;
+            KP.t0.d[k] = 1.0;
+            KP.tinf.d[k] = ps.dtinf_dt0;
+            KP.bl_u.d[k] = ps.dblu_dt0;
         }
-        OpGeoStored<Dual<kDualN>> G{&D, i};
-        OpOut<Dual<kDualN>> o;
-        cell_ops<Dual<kDualN>>(D, i, ConstLoader{qs, n}, G, KP, o);
-        double *P = JEp + (size_t)i * kOpParRows * 3;
-        for (int k = 0; k < kDualN && e * kDualN + k < 3; ++k) {
-            const int p = e * kDualN + k;
-            P[p] = o.d[0].d[k];
-            for (int m = 1; m < 5; ++m) P[m * 3 + p] = o.d[m].d[k] + o.vis[m - 1].d[k];
+        if (p == 2) {
+            KP.rtol.d[k] = 1.0;
+            KP.rtol_rtred.d[k] = ps.rtred;
         }
-        return;
     }
-    e -= kParGroups;
-    if (!JEx || e >= kNodeGroups) return;
+    OpGeoStored<Dual<kDualN>> G{&D, i};
+    OpOut<Dual<kDualN>> o;
+    cell_ops<Dual<kDualN>>(D, i, ConstLoader{qs, n}, G, KP, o);
+    double *P = JEp + (size_t)i * kOpParRows * 3;
+    for (int k = 0; k < kDualN && e * kDualN + k < 3; ++k) {
+        const int p = e * kDualN + k;
+        P[p] = o.d[0].d[k];
+        for (int m = 1; m < 5; ++m) P[m * 3 + p] = o.d[m].d[k] + o.vis[m - 1].d[k];
+    }
+}
+
+__global__ void __launch_bounds__(64) k_op_jac_xy(Dev D, Consts<double> K, const double *qs,
+                                                  double *JEx) {
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int e = blockIdx.y;
+    if (i >= D.n) return;
+    const int n = D.n;
+    if (e >= kNodeGroups) return;
+    const Consts<Dual<kDualN>> KD = lift<kDualN>(K);
     const int d0 = e * kDualN;   // first of this group's directions
     OpGeoSlots<Dual<kDualN>> GS;
     GS.D = &D;
     GS.i = i;
     const int4 nb4 = D.nc4[i];
     GS.nb[0] = nb4.x; GS.nb[1] = nb4.y; GS.nb[2] = nb4.z; GS.nb[3] = nb4.w;
-    const int *nod = reinterpret_cast<const int *>(D.nod4);
-    for (int sl = 0; sl < kNodeSlots; ++sl) {
-        int v;
-        if (sl < 4) {
-            v = nod[4 * i + sl];
-        } else {
-            const int j = GS.nb[(sl - 4) >> 2];
-            v = j < n ? nod[4 * j + ((sl - 4) & 3)] : -1;
-        }
-        GS.node[sl] = v;
-        GS.xs[sl] = Dual<kDualN>(v >= 0 ? D.xn[v] : 0.0);
-        GS.ys[sl] = Dual<kDualN>(v >= 0 ? D.yn[v] : 0.0);
-    }
-    for (int k = 0; k < kDualN && d0 + k < 2 * kNodeSlots; ++k) {   // slot (d0+k)/2, x if even
-        const int dir = d0 + k, sl = dir >> 1;
-        if (dir & 1) GS.ys[sl].d[k] = 1.0;
-        else GS.xs[sl].d[k] = 1.0;
-    }
+    GS.seed(d0);
     slots_finish(GS, K.iaxis);
     OpOut<Dual<kDualN>> o;
     cell_ops<Dual<kDualN>>(D, i, ConstLoader{qs, n}, GS, KD, o);
@@ -685,9 +748,14 @@ hipError_t launch_op_jac(tdt_ctx *c, const double *qs, const int *mptr, const in
     ps.dtinf_dt0 = (double)K.tinf / P.t0;
     ps.dblu_dt0 = 0.5 * (double)K.bl_u / P.t0;
     ps.rtred = c->params_rtred;
-    const unsigned evals = kmax * kMemberGroups + kParGroups + (JEx ? kNodeGroups : 0);
-    hipLaunchKernelGGL(k_op_jac, dim3((c->D.n + 63) / 64, evals), dim3(64), 0, c->stream, c->D,
-                       K, ps, qs, mptr, mlist, kmax, JE, JEp, JEx);
+    const unsigned nb = (c->D.n + 63) / 64;
+    hipLaunchKernelGGL(k_op_jac, dim3(nb, kmax * kMemberGroups), dim3(64), 0, c->stream, c->D, K,
+                       qs, mptr, mlist, JE);
+    hipLaunchKernelGGL(k_op_jac_par, dim3(nb, kParGroups), dim3(64), 0, c->stream, c->D, K, ps,
+                       qs, JEp);
+    if (JEx)
+        hipLaunchKernelGGL(k_op_jac_xy, dim3(nb, kNodeGroups), dim3(64), 0, c->stream, c->D, K,
+                           qs, JEx);
     return hipGetLastError();
 }
 
