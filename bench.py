@@ -43,35 +43,47 @@ DESCRIBE = {"channel_97x49": "2D turbulent channel / flat plate",
 KERNELS = ("k_pre", "k_dv", "k_stage1", "k_stage2", "k_stage3", "k_stage4")
 
 
-def alg_bytes_per_cell(precision: str, m_over_n: float, local: int, kset: str = "split") -> dict:
+def alg_bytes_per_cell(precision: str, m_over_n: float, local: int, kset: str = "split",
+                       f_over_n: float = 2.0) -> dict:
     """Algorithmic bytes per interior cell for one launch of each fused kernel.
 
     Every array a kernel reads or writes is counted once at its storage width;
-    neighbour and node re-reads are cache hits; ghost work (O(perimeter)) is
-    neglected.  s: width of stage states, D, VIS, sensors and node averages; g:
-    width of the geometry the flux arithmetic reads (FP32 copies in the mixed
-    path, whose Q update still reads the FP64 volume).  Q^n and DT are FP64.
-    The merged and quad kernel sets have no k_dv launch: stage 1 computes the
-    dissipation and viscous fluxes itself, so it moves k_dv's bytes plus its
-    stage output.
+    neighbour, node and record re-reads are cache hits; ghost work
+    (O(perimeter)) is neglected.  s: width of stage states, cell records, node
+    averages and the frozen terms; g: width of the geometry the flux arithmetic
+    reads (FP32 copies in the mixed path).  Q^n, DT, DL and the volume the
+    update divides by are FP64.  Cell record: ANU, ANURT, VOL/DT, D2Q1..5 (8
+    values); node record: UN, VN, TN, RHON, RN and the coordinates (7 values);
+    frozen terms: D and VIS (9 values), or D + VIS combined (5, FP32-mixed).
+    The merged and quad kernel sets have no k_dv launch (stage 1 forms the
+    dissipation and viscous fluxes itself); the face set's k_vface (slot k_dv)
+    evaluates each of the f_over_n faces per cell once: a 16-byte face record
+    read and a 16-byte flux written per face, which stage 1 gathers.
     """
     s = g = 8 if precision == "fp64" else 4
-    q, nc, nod, dt, dl = 40, 16, 16, 8, 8
-    sc, xy = 8 * g, 2 * g
-    vol_upd = 8 if precision != "fp64" else 0           # FP64 volume besides the FP32 copy
-    pre_cells = q + dl + nc + (dt if local else 0) + 7 * s   # -> DT, ANU, ANURT, D2Q
-    pre_nodes = m_over_n * (4 + 8 * 4 + 5 * s)               # records -> 5 averages
-    dv = (q + dt + g + 2 * s + 5 * s + nc + nod + sc + xy + m_over_n * (2 * g + 5 * s)
-          + 5 * s + 4 * s)                                   # -> D, VIS
-    common = nc + sc + xy + dt + g + vol_upd + 5 * s + 4 * s   # geometry, DT, D, VIS
-    if precision == "fp64":
-        common -= g                                          # one volume array
-    st1 = q + common + 5 * s
-    st23 = 5 * s + q + common + 5 * s
-    st4 = 5 * s + q + common + q
-    if not kset.startswith("split"):
-        dv, st1 = 0.0, dv + vol_upd + 5 * s
-    return {"k_pre": pre_cells + pre_nodes, "k_dv": dv, "k_stage1": st1, "k_stage2": st23,
+    q, dt, dl, nct, nod = 40, 8, 8, 16, 16
+    rec = 8 * s
+    nodes = m_over_n * 7 * s
+    frozen = 9 * s if precision == "fp64" else 5 * s
+    geo = 8 * g + 2 * g + 8                  # face vectors, centroids, FP64 volume
+    pre = (q + dl + nct + g + (dt if local else 0) + rec
+           + m_over_n * (4 + 8 * 4 + 5 * s))     # node: count word, 4 contributors
+    base = dt + nct + geo + frozen             # every stage: DT, ids, geometry, D (+VIS)
+    dv = dt + nct + nod + q + rec + 10 * g + nodes + frozen
+    st1 = q + base + 5 * s
+    st23 = 5 * s + q + base + 5 * s
+    st4 = 5 * s + q + base + q
+    st1_kind = "split" if kset.startswith("split") else "merged"
+    for kind in ("split", "quad", "face", "merged"):
+        if kset.startswith(kind) or f"stage1={kind}" in kset:
+            st1_kind = kind
+            break
+    if st1_kind in ("merged", "quad"):
+        dv, st1 = 0.0, q + base + nod + rec + nodes + 5 * s
+    elif st1_kind == "face":
+        dv = f_over_n * 32 + q + 10 * g + nodes
+        st1 = q + base + rec + 16 + f_over_n * 16 + 5 * s
+    return {"k_pre": pre, "k_dv": dv, "k_stage1": st1, "k_stage2": st23,
             "k_stage3": st23, "k_stage4": st4}
 
 
