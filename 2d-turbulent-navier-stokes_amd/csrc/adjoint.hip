@@ -7,8 +7,9 @@
 // is built from its parent, which is i, and for type 3 from the neighbour
 // across i's opposite face, itself one of i's neighbours).  So dR/dQ is a
 // block-sparse matrix with one 5x5 block per (cell, stencil member):
-//   k_res_jac  forms, per cell (one thread per block), the five blocks dR_i/dQ_s
-//              (s = i, then the neighbours across faces 1..4) and dR_i/d(P0, T0), by forward-
+//   k_res_jac  forms, per cell (one thread per dual direction), the five blocks
+//              dR_i/dQ_s (s = i, then the neighbours across faces 1..4), and
+//              k_res_jac_par dR_i/d(P0, T0), by forward-
 //              mode dual numbers through the same physics functions the solver
 //              runs (phys.hpp), so the value part is the solver's R bit for bit;
 //   k_res_vjp  applies the transpose as a gather: g_k = sum over the (i, s)
@@ -16,7 +17,7 @@
 //              transposed stencil, deterministic, no atomics;
 //   k_par_vjp  + k_par_fold: v^T dR/d(P0, T0) as block partial sums folded in a
 //              fixed order;
-//   k_xy_vjp   v^T dR/d(node x, y): k_res_jac also differentiates R_i through the
+//   k_xy_vjp   v^T dR/d(node x, y): k_res_jac_xy differentiates R_i through the
 //              geometry of geometry.f90 recomputed from the nodes it reads
 //              (the cell's, its face neighbours', and the ghost centroids'
 //              face nodes), and the gather runs over each node's (cell, slot)
@@ -205,63 +206,86 @@ constexpr int kMemberGroups = (5 + kDualN - 1) / kDualN;             // per memb
 constexpr int kParGroups = (2 + kDualN - 1) / kDualN;                // P0, T0
 constexpr int kNodeGroups = (2 * kNodeSlots + kDualN - 1) / kDualN;  // slot x, y
 
-// One dual evaluation per thread, blockIdx.y = e: member s = e / kMemberGroups
-// (s < 5) with components c0.. seeded (J[i] block s = dR_i,m / dQ_s,c [m][c],
-// zero for a ghost face); then the parameter groups (Jp[i] = [m][p]: dR_i,m /
-// d(P0, T0)); then the node groups (Jx[i], directions (slot, x or y)).
-__global__ void __launch_bounds__(kBlock) k_res_jac(Dev D, Consts<double> K, ParamSeeds ps,
-                                                    const double *qs, double *J, double *Jp,
-                                                    double *Jx) {
+// One dual evaluation per thread, in three kernels (each gets the registers it
+// needs alone).  k_res_jac, blockIdx.y = e: member s = e / kMemberGroups (s < 5)
+// with components c0.. seeded (J[i] block s = dR_i,m / dQ_s,c [m][c], zero for a
+// ghost face).  k_res_jac_par: parameter group e (Jp[i] = [m][p]: dR_i,m /
+// d(P0, T0)).  k_res_jac_xy: node group e (Jx[i], directions (slot, x or y)).
+__device__ __forceinline__ void res_stencil(const Dev &D, int i, const double *qs,
+                                            Dual<kDualN> (&q)[5][5]) {
+    double qd[5][5];
+    load_stencil(D, qs, i, qd);
+    for (int a = 0; a < 5; ++a)
+        for (int c = 0; c < 5; ++c) q[a][c] = Dual<kDualN>(qd[a][c]);
+}
+
+__global__ void __launch_bounds__(kBlock) k_res_jac(Dev D, Consts<double> K, const double *qs,
+                                                    double *J) {
     const int i = gid();
-    int e = blockIdx.y;
+    const int e = blockIdx.y;
     if (i >= D.n) return;
     const int n = D.n;
     const int4 nb4 = D.nc4[i];
     const int nb[4] = {nb4.x, nb4.y, nb4.z, nb4.w};
-    double qd[5][5];
-    load_stencil(D, qs, i, qd);
-    const Consts<Dual<kDualN>> KD = lift<kDualN>(K);
-    Dual<kDualN> q[5][5], r[5];
-    for (int a = 0; a < 5; ++a)
-        for (int c = 0; c < 5; ++c) q[a][c] = Dual<kDualN>(qd[a][c]);
-    if (e < kJacBlocks * kMemberGroups) {
-        const int sm = e / kMemberGroups, c0 = (e % kMemberGroups) * kDualN;
-        double *Ji = J + (size_t)i * kJacStride + sm * 25;
-        if (sm > 0 && nb[sm - 1] >= n) {
-            for (int k = 0; k < kDualN && c0 + k < 5; ++k)
-                for (int m = 0; m < 5; ++m) Ji[m * 5 + c0 + k] = 0.0;
-            return;
-        }
-        CellGeo<Dual<kDualN>> GD;
-        geo_stored(D, i, GD);
-        for (int k = 0; k < kDualN && c0 + k < 5; ++k) q[sm][c0 + k].d[k] = 1.0;
-        cell_residual<Dual<kDualN>>(D, i, q, GD, KD, r);
+    const int sm = e / kMemberGroups, c0 = (e % kMemberGroups) * kDualN;
+    double *Ji = J + (size_t)i * kJacStride + sm * 25;
+    if (sm > 0 && nb[sm - 1] >= n) {
         for (int k = 0; k < kDualN && c0 + k < 5; ++k)
-            for (int m = 0; m < 5; ++m) Ji[m * 5 + c0 + k] = r[m].d[k];
+            for (int m = 0; m < 5; ++m) Ji[m * 5 + c0 + k] = 0.0;
         return;
     }
-    e -= kJacBlocks * kMemberGroups;
-    if (e < kParGroups) {   // parameter p = e * kDualN + k in direction k
-        Consts<Dual<kDualN>> KP = KD;
-        for (int k = 0; k < kDualN; ++k) {
-            const int p = e * kDualN + k;
-            if (p == 0) KP.s0.d[k] = ps.ds0_dp0;
-            if (p == 1) {
-                KP.s0.d[k] = ps.ds0_dt0;
-                KP.t0.d[k] = 1.0;
-                KP.tinf.d[k] = ps.dtinf_dt0;
-                KP.bl_u.d[k] = ps.dblu_dt0;
-            }
+    Dual<kDualN> q[5][5], r[5];
+    res_stencil(D, i, qs, q);
+    const Consts<Dual<kDualN>> KD = lift<kDualN>(K);
+    CellGeo<Dual<kDualN>> GD;
+    geo_stored(D, i, GD);
+    // seeded by static indices (a runtime index would put q in scratch memory)
+#pragma unroll
+    for (int a = 0; a < 5; ++a)
+#pragma unroll
+        for (int c = 0; c < 5; ++c)
+#pragma unroll
+            for (int k = 0; k < kDualN; ++k)
+                if (a == sm && c == c0 + k) q[a][c].d[k] = 1.0;
+    cell_residual<Dual<kDualN>>(D, i, q, GD, KD, r);
+    for (int k = 0; k < kDualN && c0 + k < 5; ++k)
+        for (int m = 0; m < 5; ++m) Ji[m * 5 + c0 + k] = r[m].d[k];
+}
+
+__global__ void __launch_bounds__(kBlock) k_res_jac_par(Dev D, Consts<double> K, ParamSeeds ps,
+                                                        const double *qs, double *Jp) {
+    const int i = gid();
+    const int e = blockIdx.y;
+    if (i >= D.n) return;
+    Dual<kDualN> q[5][5], r[5];
+    res_stencil(D, i, qs, q);
+    // parameter p = e * kDualN + k in direction k
+    Consts<Dual<kDualN>> KP = lift<kDualN>(K);
+    for (int k = 0; k < kDualN; ++k) {
+        const int p = e * kDualN + k;
+        if (p == 0) KP.s0.d[k] = ps.ds0_dp0;
+        if (p == 1) {
+            KP.s0.d[k] = ps.ds0_dt0;
+            KP.t0.d[k] = 1.0;
+            KP.tinf.d[k] = ps.dtinf_dt0;
+            KP.bl_u.d[k] = ps.dblu_dt0;
         }
-        CellGeo<Dual<kDualN>> GD;
-        geo_stored(D, i, GD);
-        cell_residual<Dual<kDualN>>(D, i, q, GD, KP, r);
-        for (int k = 0; k < kDualN && e * kDualN + k < 2; ++k)
-            for (int m = 0; m < 5; ++m) Jp[(size_t)i * 10 + m * 2 + e * kDualN + k] = r[m].d[k];
-        return;
     }
-    e -= kParGroups;
-    if (!Jx || e >= kNodeGroups) return;
+    CellGeo<Dual<kDualN>> GD;
+    geo_stored(D, i, GD);
+    cell_residual<Dual<kDualN>>(D, i, q, GD, KP, r);
+    for (int k = 0; k < kDualN && e * kDualN + k < 2; ++k)
+        for (int m = 0; m < 5; ++m) Jp[(size_t)i * 10 + m * 2 + e * kDualN + k] = r[m].d[k];
+}
+
+__global__ void __launch_bounds__(kBlock) k_res_jac_xy(Dev D, Consts<double> K, const double *qs,
+                                                       double *Jx) {
+    const int i = gid();
+    const int e = blockIdx.y;
+    if (i >= D.n || e >= kNodeGroups) return;
+    Dual<kDualN> q[5][5], r[5];
+    res_stencil(D, i, qs, q);
+    const Consts<Dual<kDualN>> KD = lift<kDualN>(K);
     const int d0 = e * kDualN;   // node coordinates: this group's directions
     Dual<kDualN> x[kNodeSlots], y[kNodeSlots];
     for (int sl = 0; sl < kNodeSlots; ++sl) {
@@ -269,11 +293,14 @@ __global__ void __launch_bounds__(kBlock) k_res_jac(Dev D, Consts<double> K, Par
         x[sl] = Dual<kDualN>(v >= 0 ? D.xn[v] : 0.0);
         y[sl] = Dual<kDualN>(v >= 0 ? D.yn[v] : 0.0);
     }
-    for (int k = 0; k < kDualN && d0 + k < 2 * kNodeSlots; ++k) {   // slot (d0+k)/2, x if even
-        const int dir = d0 + k, sl = dir >> 1;
-        if (dir & 1) y[sl].d[k] = 1.0;
-        else x[sl].d[k] = 1.0;
-    }
+    // slot (d0 + k) / 2, x if even; seeded by static indices
+#pragma unroll
+    for (int sl = 0; sl < kNodeSlots; ++sl)
+#pragma unroll
+        for (int k = 0; k < kDualN; ++k) {
+            if (d0 + k == 2 * sl) x[sl].d[k] = 1.0;
+            if (d0 + k == 2 * sl + 1) y[sl].d[k] = 1.0;
+        }
     CellGeo<Dual<kDualN>> GX;
     geo_from_nodes(D, i, x, y, K.iaxis, GX);
     cell_residual<Dual<kDualN>>(D, i, q, GX, KD, r);
@@ -386,9 +413,13 @@ static ParamSeeds param_seeds(const tdt_ctx *c) {
 
 hipError_t launch_res_jac(tdt_ctx *c, const double *qs, double *J, double *Jp, double *Jx) {
     const unsigned nb = nblocks(c->D.n);
-    const unsigned evals = kJacBlocks * kMemberGroups + kParGroups + (Jx ? kNodeGroups : 0);
-    hipLaunchKernelGGL(k_res_jac, dim3(nb, evals), dim3(kBlock), 0, c->stream, c->D, c->K,
-                       param_seeds(c), qs, J, Jp, Jx);
+    hipLaunchKernelGGL(k_res_jac, dim3(nb, kJacBlocks * kMemberGroups), dim3(kBlock), 0,
+                       c->stream, c->D, c->K, qs, J);
+    hipLaunchKernelGGL(k_res_jac_par, dim3(nb, kParGroups), dim3(kBlock), 0, c->stream, c->D,
+                       c->K, param_seeds(c), qs, Jp);
+    if (Jx)
+        hipLaunchKernelGGL(k_res_jac_xy, dim3(nb, kNodeGroups), dim3(kBlock), 0, c->stream, c->D,
+                           c->K, qs, Jx);
     return hipGetLastError();
 }
 
