@@ -92,6 +92,16 @@ class Mesh:
                 "parent": self.parent, "ghost": self.ghost, "btype": self.btype,
                 "side": self.side}
 
+    def digest(self) -> str:
+        """SHA-256 over the mesh arrays (fixed key order, native byte layout): names
+        a generated grid in a fixture without storing it."""
+        import hashlib
+        h = hashlib.sha256()
+        for k, v in self.arrays().items():
+            h.update(k.encode())
+            h.update(np.ascontiguousarray(v).tobytes())
+        return h.hexdigest()
+
     @staticmethod
     def from_arrays(d) -> "Mesh":
         return Mesh(np.ascontiguousarray(d["x_mm"], dtype=np.float64),

@@ -26,7 +26,13 @@ class Golden:
         self.name = name
         with np.load(os.path.join(GOLDEN, name + ".npz")) as z:
             self.z = {k: z[k] for k in z.files}
-        self.mesh = caseio.Mesh.from_arrays({k[2:]: v for k, v in self.z.items() if k.startswith("m_")})
+        if "mesh_workload" in self.z:   # a generated grid, named by its digest
+            from tdt import gridgen
+            self.mesh = gridgen.WORKLOADS[str(self.z["mesh_workload"])][0]()
+            assert self.mesh.digest() == str(self.z["mesh_digest"]), name + ": grid differs"
+        else:
+            self.mesh = caseio.Mesh.from_arrays({k[2:]: v for k, v in self.z.items()
+                                                 if k.startswith("m_")})
         self.flow = caseio.Flow(**json.loads(str(self.z["flow_json"])))
         self.nrun = int(self.z["nrun"])
 

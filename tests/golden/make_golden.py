@@ -57,11 +57,13 @@ def parse_final(path):
 
 
 def run_case(name, mesh, flow, nrun, dump, sample_stride=1, crosscheck=True, big=False,
-             keep_linf=True, periodic=False):
+             keep_linf=True, periodic=False, workload=None):
     """big: the NN = 1,060,000 build of the driver and executable; keep_linf: keep the
     L-infinity history (False keeps the L2 history only); periodic: keep the head
     and tail lines of the executable's periodic restart file restart<I10>.in
-    (written every 20000 iterations, turbnstwod.f90:170-182)."""
+    (written every 20000 iterations, turbnstwod.f90:170-182); workload: the grid is
+    gridgen.WORKLOADS[workload]'s, stored as that name and the mesh digest instead
+    of its arrays."""
     work = tempfile.mkdtemp(prefix=f"tdt_golden_{name}_")
     sfx = "_big" if big else ""
     try:
@@ -82,7 +84,10 @@ def run_case(name, mesh, flow, nrun, dump, sample_stride=1, crosscheck=True, big
             agree = bool(np.array_equal(c_ref, final))
             if not agree:
                 raise SystemExit(f"{name}: driver and reference executable disagree")
-        data = {f"m_{k}": v for k, v in mesh.arrays().items()}
+        if workload:
+            data = {"mesh_workload": np.array(workload), "mesh_digest": np.array(mesh.digest())}
+        else:
+            data = {f"m_{k}": v for k, v in mesh.arrays().items()}
         data["flow_json"] = np.array(json.dumps(dataclasses.asdict(flow)))
         data["nrun"] = np.array(nrun)
         data["hist_l2"] = l2
@@ -213,7 +218,10 @@ def run_host_case(name, case_dir, niter):
                                                                  mesh.neles)
         head, nodes, elems = parse_tecplot(os.path.join(work, "tec_plot.dat"), mesh.nods,
                                            mesh.neles)
-        data = {f"m_{k}": v for k, v in mesh.arrays().items()}
+        if workload:
+            data = {"mesh_workload": np.array(workload), "mesh_digest": np.array(mesh.digest())}
+        else:
+            data = {f"m_{k}": v for k, v in mesh.arrays().items()}
         data.update({
             "flow_json": np.array(json.dumps(dataclasses.asdict(flow))), "nrun": np.array(niter),
             "stdout": np.array(out), "fort32": np.array(open(os.path.join(work, "fort.32")).read()),
@@ -309,7 +317,7 @@ def main():
     # goes non-finite (about 45 minutes on one core)
     m, f = gridgen.WORKLOADS["ramp_2049x513"][0](), gridgen.WORKLOADS["ramp_2049x513"][1]()
     run_case("ramp_2049x513_t3", m, f, nrun=3720, dump=False, sample_stride=4099,
-             crosscheck=False, big=True)
+             crosscheck=False, big=True, workload="ramp_2049x513")
 
 
 if __name__ == "__main__":

@@ -182,3 +182,30 @@ def test_ramp_type3_divergence_follows_perturbation_size(ramp_t3):
     assert bad[6e-8] < bad[1e-9] < bad[0.0]
     assert bad[6e-8] < 0.6 * bad[0.0]
     assert abs(bad["fp32-mixed"] - bad[6e-8]) < 0.25 * bad[6e-8], bad
+
+
+def test_ramp_type3_blowup_location(ramp_t3):
+    """Where the type-3 ramp blows up: in both precisions the first non-finite
+    cells lie in the inflow column (i = 0, the cells next to it reached by the
+    same iteration's four stages) next to the top boundary, where the
+    extrapolated inflow meets the extrapolated top (cell id = i * 513 + j)."""
+    import tdt
+    mesh, flow = ramp_t3
+    ny = 513
+    for prec, start in (("fp32-mixed", 1400), ("fp64", 3600)):
+        with tdt.Solver(mesh, flow, prec) as s:
+            s.cold_start()
+            assert s.iterate(start, history=False, check=False)[2] == 0, prec
+            it = start
+            while it < start + 400:
+                it += 1
+                if s.iterate(1, history=False, check=False)[2]:
+                    break
+            c = s.get_state()
+        cells = np.flatnonzero(~np.isfinite(c).all(axis=0))
+        i, j = cells // ny, cells % ny
+        print(f"{prec}: non-finite at iteration {it} in {len(cells)} cells, columns "
+              f"{i.min()}..{i.max()}, rows {j.min()}..{j.max()}")
+        assert len(cells) > 0, prec
+        assert i.min() == 0 and i.max() <= 4 and j.min() >= 480, (prec, i.max(), j.min())
+
