@@ -429,8 +429,10 @@ hipError_t launch_res_apply(tdt_ctx *c, const double *J, const double *Jp, const
     const unsigned nb = nblocks(c->D.n);
     hipLaunchKernelGGL(k_res_vjp, dim3(nb), dim3(kBlock), 0, c->stream, c->D, J, v, tptr, tlist,
                        g);
-    hipLaunchKernelGGL(k_par_vjp, dim3(nb), dim3(kBlock), 0, c->stream, c->D, Jp, v, part);
-    hipLaunchKernelGGL(k_par_fold, dim3(1), dim3(kWave), 0, c->stream, part, (int)nb, gp);
+    if (gp) {   // gp == nullptr: the state gradient only
+        hipLaunchKernelGGL(k_par_vjp, dim3(nb), dim3(kBlock), 0, c->stream, c->D, Jp, v, part);
+        hipLaunchKernelGGL(k_par_fold, dim3(1), dim3(kWave), 0, c->stream, part, (int)nb, gp);
+    }
     return hipGetLastError();
 }
 

@@ -767,9 +767,11 @@ hipError_t launch_op_vjp(tdt_ctx *c, const double *JE, const double *JEp, const 
     const unsigned nb = nblocks(c->D.n);
     hipLaunchKernelGGL(k_op_vjp, dim3(nb), dim3(kBlock), 0, c->stream, c->D, JE, W, gdt, tptr,
                        tlist, g);
-    hipLaunchKernelGGL(k_op_par_vjp, dim3(nb), dim3(kBlock), 0, c->stream, c->D, JEp, W, grtol,
-                       part);
-    hipLaunchKernelGGL(k_op_par_fold, dim3(1), dim3(kWave), 0, c->stream, part, (int)nb, gp);
+    if (gp) {   // gp == nullptr: the state gradient only
+        hipLaunchKernelGGL(k_op_par_vjp, dim3(nb), dim3(kBlock), 0, c->stream, c->D, JEp, W,
+                           grtol, part);
+        hipLaunchKernelGGL(k_op_par_fold, dim3(1), dim3(kWave), 0, c->stream, part, (int)nb, gp);
+    }
     if (gxy && JEx)
         hipLaunchKernelGGL(k_op_xy_vjp, dim3(nblocks(c->D.m)), dim3(kBlock), 0, c->stream, c->D,
                            JEx, W, gdt, gvol, nptr, nlist, gxy);

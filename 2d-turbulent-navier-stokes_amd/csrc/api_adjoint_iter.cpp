@@ -259,22 +259,23 @@ int frozen_vjp(tdt_ctx *c, const double *qn) {
     HIPCALL(c, zero(c, b.lam, n5), "adjoint: clear");
     HIPCALL(c, zero(c, b.W, n5), "adjoint: clear");
     HIPCALL(c, zero(c, b.gdt, n), "adjoint: clear");
-    HIPCALL(c, zero(c, b.gvol, n), "adjoint: clear");
-    HIPCALL(c, zero(c, b.grtol, n), "adjoint: clear");
+    // b.gvol and b.grtol collect the stage reverses' volume and RTOL terms, which
+    // nothing here reads (iteration_vjp_core clears them before it uses them);
+    // the residual and operator reverses skip their parameter sums (gp nullptr)
+    const double *mu = b.mu;   // stage k's adjoint input: b.mu, then the previous a.g
     for (int k = 3; k >= 0; --k) {
         HIPCALL(c, tdt::launch_stage_vjp(c, kAlphas[k], qn, b.r[k], D.vis, D.d24, D.dt, D.vol,
-                                         b.mu, b.lam, b.wr, b.W, b.gdt, b.gvol, b.grtol),
+                                         mu, b.lam, b.wr, b.W, b.gdt, b.gvol, b.grtol),
                 "adjoint: stage update");
         HIPCALL(c, tdt::launch_res_apply(c, b.Jst[k], b.Jpst[k], b.wr, a.tptr, a.tlist, a.part,
-                                         a.g, a.gp), "adjoint: residual");
+                                         a.g, nullptr), "adjoint: residual");
         if (k > 0)
-            HIPCALL(c, hipMemcpyAsync(b.mu, a.g, n5 * sizeof(double), hipMemcpyDeviceToDevice,
-                                      c->stream), "adjoint: copy");
+            mu = a.g;   // the stream orders the next stage's read before a.g is rewritten
         else
             HIPCALL(c, tdt::launch_add(c, (int)n5, a.g, b.lam), "adjoint: accumulate");
     }
     HIPCALL(c, tdt::launch_op_vjp(c, b.JE, b.JEp, nullptr, b.W, b.gdt, b.gvol, b.grtol, b.tptr,
-                                  b.tlist, a.nptr, a.nlist, b.part, b.g, b.gpx, nullptr),
+                                  b.tlist, a.nptr, a.nlist, b.part, b.g, nullptr, nullptr),
             "adjoint: operators");
     HIPCALL(c, tdt::launch_add(c, (int)n5, b.g, b.lam), "adjoint: accumulate");
     return TDT_OK;
@@ -488,7 +489,7 @@ int tdt_adjoint_fixed_point(tdt_ctx *c, const double *cst, double rinit, const d
     double rel = INFINITY, prev = INFINITY;
     while (!rc && it < max_iter && !(rel <= tol)) {
         const bool check = (it + 1) % kCheck == 0 || it + 1 == max_iter;
-        if (check)   // lam_k (frozen_vjp reuses b.mu for the stages' adjoints)
+        if (check)   // lam_k
             HIPCALL(c, hipMemcpyAsync(h0.data(), b.mu, n5 * sizeof(double),
                                       hipMemcpyDeviceToHost, c->stream), "download");
         rc = frozen_vjp(c, b.qn);
@@ -522,8 +523,7 @@ int tdt_adjoint_fixed_point(tdt_ctx *c, const double *cst, double rinit, const d
             if (it == max_iter) rel = std::isfinite(rel) ? rel : d;
             prev = d;
         }
-        HIPCALL(c, hipMemcpyAsync(b.mu, b.lam, n5 * sizeof(double), hipMemcpyDeviceToDevice,
-                                  c->stream), "adjoint: copy");
+        std::swap(b.mu, b.lam);   // lam_{k+1} is the next iteration's input
     }
     // lam* out, then one full reverse iteration with mu = lam* for Phi_p^T lam*
     // and Phi_xy^T lam*

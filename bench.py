@@ -343,7 +343,7 @@ def small_case(stream_gbs, converge=2000, cpu=True):
 
 
 def batch_sweep(ncases=64, steps=100, warmup=20, seq_steps=50, precision="fp32-mixed",
-                sequential=True, ordering="auto"):
+                sequential=True, ordering="auto", cpu=True):
     """The batched parameter sweep (gridgen.sweep_cases: 64 cases of both
     families, uneven sizes): all cases in one batch context, one fused iteration
     per step over the concatenated cells, against the same cases run one after
@@ -393,6 +393,8 @@ def batch_sweep(ncases=64, steps=100, warmup=20, seq_steps=50, precision="fp32-m
     out["sequential"] = {"value": round(seq_rate, 2), "unit": "Mcell-iters/s",
                          "timed_steps_per_case": seq_steps, "nonfinite_iter": int(seq_bad)}
     out["speedup"] = round(batched / seq_rate, 3)
+    if not cpu:
+        return out
     # the host's cores on the same cases (a process of its own: its workers fork,
     # and a process that has used the GPU must not)
     r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-sweep", str(ncases),
@@ -589,15 +591,60 @@ def _cpu_sweep_worker(idx, iters, barrier, out):
     out.put((sum(n for _, n in orcs) * iters, time.perf_counter() - t0))
 
 
+def _reference_for(mesh):
+    """The smallest reference build whose NN holds the grid's cells + ghosts."""
+    for name, nn in REF_BUILDS:
+        path = os.path.join(REPO, "oracle", "_ref", name)
+        if mesh.neles + mesh.nghosts <= nn and os.path.exists(path):
+            return path
+    return None
+
+
+def _cpu_sweep_ref_worker(idx, iters, barrier, out):
+    """One host core: its share of the sweep's cases run by the reference
+    executable.  Each case is written once; its loop time is t(NITER=iters) -
+    t(NITER=0), so reading the case and writing the results cancel out."""
+    if hasattr(os, "sched_getaffinity"):   # one allowed core per worker
+        allowed = sorted(os.sched_getaffinity(0))
+        os.sched_setaffinity(0, {allowed[idx[0] % len(allowed)]})
+    work = tempfile.mkdtemp(prefix="tdt_cpu_sweep_")
+    try:
+        runs = []
+        for i in idx[1]:
+            _, m, f = _SWEEP[i]
+            d = os.path.join(work, str(i))
+            os.mkdir(d)
+            caseio.write_case(d, caseio.Flow(**{**f.__dict__, "niter": 0, "istart": 0}), m)
+            runs.append((d, f, m.neles, _reference_for(m)))
+
+        def run(d, f, exe, niter):
+            caseio.write_flow(os.path.join(d, "flow.in"),
+                              caseio.Flow(**{**f.__dict__, "niter": niter, "istart": 0}))
+            t0 = time.perf_counter()
+            subprocess.run([exe], cwd=d, stdout=subprocess.DEVNULL, check=True)
+            return time.perf_counter() - t0
+        barrier.wait()
+        loop = 0.0
+        for d, f, _, exe in runs:
+            t_0 = run(d, f, exe, 0)
+            loop += max(run(d, f, exe, iters) - t_0, 0.0)
+        out.put((sum(n for _, _, n, _ in runs) * iters, loop))
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
 def cpu_sweep(ncases: int, iters: int, cores: int):
-    """The batched sweep's cases on the host's cores: the CPU oracle (the
-    reference's arithmetic restated in C, FP64, one process per core), cases
-    spread over the cores by longest-processing-time (cell count), each core
-    iterating its cases `iters` times after all have set up.  Aggregate =
-    total cell-iterations / the slowest core's loop time."""
+    """The batched sweep's cases on the host's cores, one process per core,
+    cases spread over the cores by longest-processing-time (cell count): the
+    reference executable itself (oracle/_ref, built from the reference's
+    sources) when it is built, else the CPU oracle (the reference's arithmetic
+    restated in C, FP64), each core iterating its cases `iters` times after all
+    have set up.  Aggregate = total cell-iterations / the slowest core's loop
+    time."""
     import multiprocessing as mp
     global _SWEEP
     _SWEEP = gridgen.sweep_cases(ncases)
+    use_ref = all(_reference_for(m) for _, m, _ in _SWEEP)
     sizes = [m.neles for _, m, _ in _SWEEP]
     load = [0] * cores
     share = [[] for _ in range(cores)]
@@ -608,7 +655,8 @@ def cpu_sweep(ncases: int, iters: int, cores: int):
     ctx = mp.get_context("fork")
     barrier = ctx.Barrier(cores)
     out = ctx.Queue()
-    procs = [ctx.Process(target=_cpu_sweep_worker, args=((w, share[w]), iters, barrier, out))
+    worker = _cpu_sweep_ref_worker if use_ref else _cpu_sweep_worker
+    procs = [ctx.Process(target=worker, args=((w, share[w]), iters, barrier, out))
              for w in range(cores)]
     for p in procs:
         p.start()
@@ -625,11 +673,13 @@ def cpu_sweep(ncases: int, iters: int, cores: int):
                 break
     except OSError:
         pass
+    what = ("reference turbnstwod / turbnstwod_big (FP64, amdflang -O3), loop time "
+            "t(NITER=k) - t(NITER=0) per case" if use_ref else "C oracle (FP64 restatement)")
     return {"value": round(work / slowest / 1e6, 3), "unit": "Mcell-iters/s", "cores": cores,
-            "kind": "port", "nproc": os.cpu_count(), "cpu": model,
+            "kind": "reference" if use_ref else "port", "nproc": os.cpu_count(), "cpu": model,
             "single_core": round(work / sum(r[1] for r in res) / 1e6, 4),
-            "sample": f"C oracle (FP64 restatement), {ncases} cases, {iters} iterations each, "
-                      f"LPT over {cores} processes; slowest core {slowest:.2f} s"}
+            "sample": f"{what}, {ncases} cases, {iters} iterations each, LPT over {cores} "
+                      f"processes; slowest core {slowest:.2f} s"}
 
 
 def copy_probe(mib: int) -> None:
@@ -713,7 +763,8 @@ def main():
     if args.batch:
         out = batch_sweep(args.batch, steps=args.steps if args.steps != 2000 else 100,
                           warmup=min(args.warmup, 20), sequential=not args.no_sequential,
-                          ordering=args.ordering, precision=args.precision or "fp32-mixed")
+                          ordering=args.ordering, precision=args.precision or "fp32-mixed",
+                          cpu=not args.no_cpu)
         print(json.dumps({"metric": "batched sweep, aggregate Mcell-iters/s on 1 MI355X",
                           **out}), flush=True)
         return
