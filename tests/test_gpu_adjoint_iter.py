@@ -582,9 +582,9 @@ def test_fixed_point_adjoint_at_a_converged_nonuniform_flow(golden):
     uniform).  J = outlet mass flow.  The steady state's dJ/dP0 and dJ/dT0 from
     lam^T dPhi/dp (the cold start's RINIT = f(P0, T0) chained in) against
     central differences (relative step 1e-6) of converged runs of this library
-    and of the reference solver (tests/golden: its converged runs at P0, T0
-    moved by +-1e-6), within 1e-4.  The converged state's J equals the
-    reference's to 1e-9."""
+    (the five runs as one batch) and of the reference solver (tests/golden: its
+    converged runs at P0, T0 moved by +-1e-6), within 1e-4.  The converged
+    state's J equals the reference's to 1e-9."""
     import dataclasses as dc
     import tdt
     from tdt import objective
@@ -596,33 +596,35 @@ def test_fixed_point_adjoint_at_a_converged_nonuniform_flow(golden):
     def J(c):
         return float(np.sum(c[1, out[:, 0]] * sx))
 
+    # the converged state and the four moved runs as one batch (five cases in one
+    # fused iteration; each case's state is that of a context of its own, bit for bit)
+    moved = [(par, sg) for par in ("p0", "t0") for sg in (1.0, -1.0)]
+    cases = [(mesh, flow)] + [(mesh, dc.replace(flow, **{par: getattr(flow, par) * (1.0 + sg * 1e-6)}))
+                              for par, sg in moved]
+    with tdt.Batch(cases) as b:
+        b.cold_start()
+        l2a, _, bad = b.iterate(50)
+        assert bad == 0
+        b.iterate(n - 51, history=False)
+        l2b, _, bad = b.iterate(1)
+        assert bad == 0
+        drop = float(np.max(l2b[-1, 0] / l2a[:, 0].max(axis=0)))
+        c = b.get_state(0)
+        jm = {key: J(b.get_state(k + 1)) for k, key in enumerate(moved)}
+    j = J(c)
+    assert drop <= 1e-8, drop
+    assert abs(j - float(g["J_base"])) <= 1e-9 * abs(float(g["J_base"])), (j, g["J_base"])
+    fd = {par: (jm[(par, 1.0)] - jm[(par, -1.0)]) / (2e-6 * getattr(flow, par))
+          for par in ("p0", "t0")}
     with tdt.Solver(mesh, flow) as s:
         rinit = s.cold_start()
-        l2, _, bad = s.iterate(n)
-        assert bad == 0
-        drop = float(np.max(l2[-1] / l2[:50].max(axis=0)))
-        c = s.get_state()
-        j = J(c)
-        assert drop <= 1e-8, drop
-        assert abs(j - float(g["J_base"])) <= 1e-9 * abs(float(g["J_base"])), (j, g["J_base"])
         dj = np.zeros_like(c)
         np.add.at(dj[1], out[:, 0], sx)
         _, gp, iters, est = s.adjoint_fixed_point(c, rinit, dj, max_iter=1500000, tol=1e-8)
         assert est <= 1e-8, (iters, est)
-        # RINIT = AMUINIT / RHOINIT of the cold start: proportional to T0^1.6 / P0
-        adj = {"p0": gp[0] - gp[2] * rinit / flow.p0,
-               "t0": gp[1] + gp[2] * 1.6 * rinit / flow.t0}
-        fd = {}
-        for par in ("p0", "t0"):
-            base = getattr(flow, par)
-            jj = []
-            for sg in (1.0, -1.0):
-                s.set_flow(dc.replace(flow, **{par: base * (1.0 + sg * 1e-6)}))
-                s.cold_start()
-                s.iterate(n, history=False)
-                jj.append(J(s.get_state()))
-            s.set_flow(flow)
-            fd[par] = (jj[0] - jj[1]) / (2e-6 * base)
+    # RINIT = AMUINIT / RHOINIT of the cold start: proportional to T0^1.6 / P0
+    adj = {"p0": gp[0] - gp[2] * rinit / flow.p0,
+           "t0": gp[1] + gp[2] * 1.6 * rinit / flow.t0}
     ref = {par: (float(g[f"J_{par}+1e-06"]) - float(g[f"J_{par}-1e-06"]))
            / (2e-6 * getattr(flow, par)) for par in ("p0", "t0")}
     print(f"converged to an L2 drop of {drop:.2e}; adjoint {iters} iterations (error estimate "
