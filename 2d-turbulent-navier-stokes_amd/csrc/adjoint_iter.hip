@@ -13,13 +13,14 @@
 // of cells around i: the face neighbours, their face neighbours (the sensors
 // and D2Q of the dissipation), and the cells and ghost parents of the node
 // averages at i's four nodes.  The host lists these members per cell.
-//   k_op_jac   forms, per cell and member (one thread each), the 6x5 block
-//              d(E_i, DT_i)/dQ_member
+//   k_op_jac   forms, per cell and member (one thread per dual direction), the
+//              6x5 block d(E_i, DT_i)/dQ_member
 //              by forward-mode dual numbers through the same arithmetic as the
 //              routine kernels (k_timestep, k_dissip1/2, k_prim, k_bc,
 //              k_nodeavg, k_viscous), so the value part is theirs bit for bit;
-//              also dE_i/d(P0, T0, RTOL) and d(E_i, DT_i, VOL_i)/d(node x, y)
-//              over the residual adjoint's 20 node slots;
+//              k_op_jac_par dE_i/d(P0, T0, RTOL) and k_op_jac_xy
+//              d(E_i, DT_i, VOL_i)/d(node x, y) over the residual adjoint's 20
+//              node slots;
 //   k_op_vjp   the transposed gather over the members (deterministic, no
 //              atomics), k_op_par_vjp / k_op_par_fold the parameter sums, and
 //              k_op_xy_vjp the node gather;
