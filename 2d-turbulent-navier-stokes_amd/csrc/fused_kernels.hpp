@@ -136,11 +136,13 @@ __device__ __forceinline__ R rec_vbydt(const SD (&r)[kRecLen], int local, double
 // The dissipation D of cell i (solver_state.f90:183-223) from the records of i
 // and its four face neighbours (a ghost neighbour is a copy of the cell) and
 // the FP64 states: returns D1..D5 in d (2nd-difference minus 4th-difference
-// sums, each accumulated in face order).
-template <class R, class SD>
+// sums, each accumulated in face order).  KEEP: also returns the neighbours'
+// states, converted to R, in qk (a ghost's is the cell's own), for a caller
+// that needs them next (stage 1's flux faces read the same Q^n).
+template <class R, class SD, bool KEEP = false>
 __device__ __forceinline__ void dissipation_rec(const FDev &F, int i, const int (&nb)[4],
                                                 const double (&ci)[5], int local, double dti,
-                                                R (&d)[5]) {
+                                                R (&d)[5], R (*qk)[5] = nullptr) {
     const int nt = F.nt, n = F.n;
     const double *Q = F.qn;
     SD ri[kRecLen];
@@ -158,7 +160,9 @@ __device__ __forceinline__ void dissipation_rec(const FDev &F, int i, const int 
         jst_coeffs_v<R>((R)ri[kRecAnurt], (R)rj[kRecAnurt], volbydt, vbn, b2, b4);
 #pragma unroll
         for (int q = 0; q < 5; ++q) {
-            const R t2 = (q < 4 ? a2 : b2) * (R)(Q[q * nt + j] - ci[q]);
+            const double qj = Q[q * nt + j];
+            if constexpr (KEEP) qk[f][q] = (R)qj;
+            const R t2 = (q < 4 ? a2 : b2) * (R)(qj - ci[q]);
             const R t4 = (q < 4 ? a4 : b4) * ((R)rj[kRecD2q + q] - (R)ri[kRecD2q + q]);
             d2[q] = f == 0 ? t2 : d2[q] + t2;
             d4[q] = f == 0 ? t4 : d4[q] + t4;
@@ -719,13 +723,22 @@ __global__ void __launch_bounds__(kBlock) k_rk(FDev F, Consts<double> K, Consts<
         }
         double dd[5], vs[4] = {0.0, 0.0, 0.0, 0.0};
         R dsum[5];
+        constexpr bool kPreload = PL || sizeof(R) == 4;
+        // stage 1 with D inline: the dissipation's neighbour states (Q^n, the
+        // stage input) double as the preloaded flux states
+        constexpr bool kKeepQ = kInlineDV && kPreload;
+        R qf[4][5];
+        double ci[5];   // stage 1 with D inline: Q^n of the cell, which is also the
+                        // stage input and COLD (loaded once for all three)
         if (kInlineDV) {
             // dissipation faces; a ghost neighbour is a copy of this cell
-            double ci[5];
 #pragma unroll
             for (int q = 0; q < 5; ++q) ci[q] = Q[q * nt + i];
             const int nbr[4] = {nb4.x, nb4.y, nb4.z, nb4.w};
-            dissipation_rec<R, SD>(F, i, nbr, ci, K.local, dti, dsum);
+            if constexpr (kKeepQ)
+                dissipation_rec<R, SD, true>(F, i, nbr, ci, K.local, dti, dsum, qf);
+            else
+                dissipation_rec<R, SD>(F, i, nbr, ci, K.local, dti, dsum);
             // stored now in either form (FP32-mixed overwrites it with D + VIS
             // after the faces): the store keeps the face loop's loads after it
 #pragma unroll
@@ -747,19 +760,25 @@ __global__ void __launch_bounds__(kBlock) k_rk(FDev F, Consts<double> K, Consts<
         // loads this cell again), so no load waits on a branch.  FP64 loads
         // them face by face unless PL: holding all twenty costs the second
         // wavefront per SIMD (256 VGPRs), which only grids too small to give a
-        // SIMD two wavefronts can spare.
-        constexpr bool kPreload = PL || sizeof(R) == 4;
-        R qf[4][5];
+        // SIMD two wavefronts can spare.  (Stage 1 with D inline has them from
+        // the dissipation already.)
+        if constexpr (!kKeepQ) {
 #pragma unroll
-        for (int f = 0; f < 4; ++f) {
-            const int jn = sel4(f, nb4.x, nb4.y, nb4.z, nb4.w);
-            const int j = jn < n ? jn : i;
+            for (int f = 0; f < 4; ++f) {
+                const int jn = sel4(f, nb4.x, nb4.y, nb4.z, nb4.w);
+                const int j = jn < n ? jn : i;
 #pragma unroll
-            for (int q = 0; q < 5; ++q) qf[f][q] = kPreload ? (R)ld<SI>(qin, q * sin + j) : R(0);
+                for (int q = 0; q < 5; ++q)
+                    qf[f][q] = kPreload ? (R)ld<SI>(qin, q * sin + j) : R(0);
+            }
         }
         Prim<R> pi;
-        prim_from_q<R>(pi, (R)ld<SI>(qin, i), (R)ld<SI>(qin, sin + i), (R)ld<SI>(qin, 2 * sin + i),
-                       (R)ld<SI>(qin, 3 * sin + i), (R)ld<SI>(qin, 4 * sin + i), KR);
+        if constexpr (kKeepQ)
+            prim_from_q<R>(pi, (R)ci[0], (R)ci[1], (R)ci[2], (R)ci[3], (R)ci[4], KR);
+        else
+            prim_from_q<R>(pi, (R)ld<SI>(qin, i), (R)ld<SI>(qin, sin + i),
+                           (R)ld<SI>(qin, 2 * sin + i), (R)ld<SI>(qin, 3 * sin + i),
+                           (R)ld<SI>(qin, 4 * sin + i), KR);
         R acc[5];
         R vu = R(0), vv = R(0), vt = R(0), vr = R(0);
         SrcNbr<R> sa{}, sb{};
@@ -871,7 +890,7 @@ __global__ void __launch_bounds__(kBlock) k_rk(FDev F, Consts<double> K, Consts<
         if (K.iaxis == 1) r[2] = r[2] - vol * (double)s3;
         const double a = alpha * fdiv(dti, vol);
 #pragma unroll
-        for (int q = 0; q < 5; ++q) cold[q] = Q[q * nt + i];
+        for (int q = 0; q < 5; ++q) cold[q] = kKeepQ ? ci[q] : Q[q * nt + i];
         cnew[0] = cold[0] - a * (r[0] - dd[0]);
         cnew[1] = cold[1] - a * (r[1] - vs[0] - dd[1]);
         cnew[2] = cold[2] - a * (r[2] - vs[1] - dd[2]);
