@@ -450,6 +450,7 @@ int tdt_set_array(tdt_ctx *c, const char *name, const double *in, int32_t len) {
     const std::string nm(name);
     if (nm == "XCC" || nm == "YCC" || nm == "VOL" || nm == "DL" || nm.rfind("SC", 0) == 0)
         ++c->geom_version;   // the fused path refreshes its copies
+    if (nm.rfind("SC", 0) == 0) c->sc_from_nodes = false;   // no longer the nodes' vectors
     return sync(c);
 }
 

@@ -134,6 +134,8 @@ struct tdt_ctx {
     int profiling = 0;
     double ktimes[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     int geom_version = 0;     // bumped when tdt_set_array changes geometry
+    bool sc_from_nodes = false;   // SC holds the node coordinates' face vectors (set by
+                                  // the geometry pass, cleared when tdt_set_array writes SC)
     double *post = nullptr;   // [6][m] POSTPROCESS output, allocated on first use
     tdt::BatchInfo *batch = nullptr;   // tdt_batch_create contexts only
     tdt::AdjBufs *adj = nullptr;       // residual adjoint buffers, allocated on first use

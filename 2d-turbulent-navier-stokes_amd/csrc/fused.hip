@@ -464,6 +464,14 @@ int fused_iterate(tdt_ctx *c, int niter, double *l2, double *linf, int *first_no
             c->err = std::string("geometry copies: ") + hipGetErrorString(e);
             return TDT_ERR_DEVICE;
         }
+        // FP64 kernels that gather a face's nodes form its surface vector from
+        // them while SC is the nodes' (the graphs hold F: captured again)
+        const int scn = fp64 && c->sc_from_nodes ? (c->params.iaxis == 1 ? 2 : 1) : 0;
+        if (scn != F.sc_nodes) {
+            (void)hipStreamSynchronize(c->stream);
+            drop_graphs(B);
+            F.sc_nodes = scn;
+        }
         B.geom_version = c->geom_version;
     }
     const size_t nrec = F.ncases ? (size_t)F.ncases : 1;   // history records per iteration

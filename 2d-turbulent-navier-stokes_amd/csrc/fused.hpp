@@ -49,6 +49,9 @@ struct FDev {
     // geometry as the flux arithmetic reads it, in the storage type SD: the FP64
     // arrays themselves, or FP32 copies rounded from them (FP32-mixed)
     const void *gxcc, *gycc, *gvol, *gsc, *gxn, *gyn;
+    int sc_nodes;                   // FP64: the SC arrays are the node coordinates' face
+                                    // vectors (1 planar, 2 axisymmetric), so kernels that
+                                    // gather the nodes may form them; 0: read SC
     double *qn;                     // [5][nt] Q^n (the C array), FP64
     void *st[2];                    // [5][n] stage states Q(1), Q(2)/Q(3)
     double *dt;                     // [n]

@@ -172,6 +172,35 @@ __device__ __forceinline__ void dissipation_rec(const FDev &F, int i, const int 
     for (int q = 0; q < 5; ++q) d[q] = d2[q] - d4[q];
 }
 
+// Surface vector of face f of cell i, from node bn to node dn.  FP64 forms it
+// from the two nodes' coordinates with COMPUTE_GEOMETRY's operations
+// (geometry.f90:77-97), which gives the SC arrays' bits without streaming them,
+// while those arrays are the nodes' (F.sc_nodes: 1 planar, 2 axisymmetric);
+// after tdt_set_array has written SC, and in FP32-mixed (whose FP32 node
+// coordinates would lose the short wall faces' length), it reads SC.  NODES:
+// the caller gathers the face's nodes anyway (k_dv).  (The stage kernels read
+// SC: their flux faces would wait on the node ids before the node gathers,
+// which costs more on latency-bound grids than the streamed SC.)
+template <class R, class SD, bool NODES>
+__device__ __forceinline__ void face_sc(const FDev &F, int f, int i, int bn, int dn, R &sx, R &sy) {
+    if constexpr (NODES && sizeof(SD) == 8) {
+        if (F.sc_nodes) {
+            const double2 b = static_cast<const double2 *>(F.nxy)[bn];
+            const double2 d = static_cast<const double2 *>(F.nxy)[dn];
+            if (F.sc_nodes == 2) {
+                sx = (d.y - b.y) * (b.y + d.y) * 0.5;
+                sy = -((d.x - b.x) * (b.y + d.y) * 0.5);
+            } else {
+                sx = d.y - b.y;
+                sy = -(d.x - b.x);
+            }
+            return;
+        }
+    }
+    sx = (R)ld<SD>(F.gsc, (size_t)(2 * f) * F.n + i);
+    sy = (R)ld<SD>(F.gsc, (size_t)(2 * f + 1) * F.n + i);
+}
+
 // The viscous terms of face f of cell A (viscous.f90:130-217): B, D = the
 // face's nodes (their records: averages and coordinates), C = the neighbour.
 template <class R, class SD>
@@ -607,10 +636,11 @@ __global__ void __launch_bounds__(kBlock) k_dv(FDev F, Consts<double> K, Consts<
             }
         }
         R rc = fdiv(c5c, rhoc);
-        ViscFace<R> o = viscous_face_rec<R, SD>(
-            F, (R)ld<SD>(F.gsc, (2 * f) * n + i), (R)ld<SD>(F.gsc, (2 * f + 1) * n + i), xa, ya,
-            ua, va, ta, rhoa, ra, bnode, dnode, (R)ld<SD>(F.gxcc, c), (R)ld<SD>(F.gycc, c), uc,
-            vc, tc, rhoc, rc, KR);
+        R sx, sy;
+        face_sc<R, SD, true>(F, f, i, bnode, dnode, sx, sy);
+        ViscFace<R> o = viscous_face_rec<R, SD>(F, sx, sy, xa, ya, ua, va, ta, rhoa, ra, bnode,
+                                                dnode, (R)ld<SD>(F.gxcc, c),
+                                                (R)ld<SD>(F.gycc, c), uc, vc, tc, rhoc, rc, KR);
         vu = vu + o.u0 + o.u1;
         vv = vv + o.v0 + o.v1;
         vt = vt + o.t0 + o.t1;

@@ -430,6 +430,7 @@ __global__ void __launch_bounds__(kBlock) k_monitor_final(const double *part, in
 hipError_t launch_geometry(tdt_ctx *c, int iaxis) {
     LAUNCH(k_geometry, c->D.n, c->D, iaxis);
     LAUNCH(k_geometry_ghosts, c->D.g, c->D);
+    c->sc_from_nodes = true;
     return hipSuccess;
 }
 

@@ -244,6 +244,44 @@ def test_autotune_is_transparent(golden, monkeypatch):
         assert np.array_equal(out[None][1], out["merged"][1])
 
 
+def _routine_iterations(s, k):
+    for _ in range(k):
+        s.timestep()
+        s.dissipation()
+        for st in range(4):
+            s.primitives()
+            s.apply_boundary()
+            if st == 0:
+                s.viscous()
+            s.source_terms()
+            s.residual()
+            s.update(ALPHAS[st])
+        s.monitor()
+
+
+@pytest.mark.parametrize("case", ["t2d_ramp", "case1_axi"])
+def test_split_set_surface_vectors_follow_set_array(golden, case, monkeypatch):
+    """FP64 k_dv forms each face's surface vector from its nodes while SC is the
+    nodes' (planar and axisymmetric forms); once tdt_set_array writes SC it must
+    read SC again, like the routine path: both stay bitwise equal."""
+    import tdt
+    monkeypatch.setenv("TDT_KERNELS", "split")
+    g = golden(case)
+    with tdt.Solver(g.mesh, g.flow, "fp64") as a, tdt.Solver(g.mesh, g.flow, "fp64") as b:
+        a.cold_start()
+        b.cold_start()
+        _routine_iterations(a, 3)
+        b.iterate(3)
+        assert np.array_equal(a.get_state(), b.get_state())
+        for s in (a, b):
+            s.set_array("SC2X", s.get_array("SC2X") * 1.01)
+            s.set_array("SC3Y", s.get_array("SC3Y") * 0.99)
+        _routine_iterations(a, 5)
+        b.iterate(5)
+        sa, sb = a.get_state(), b.get_state()
+        assert np.array_equal(sa, sb), f"max diff {np.max(np.abs(sa - sb))}"
+
+
 def test_fp32_geometry_copies_follow_set_array(golden):
     """The FP32-mixed kernels read FP32 copies of the geometry; changing the
     geometry through tdt_set_array after a first iterate must reach them."""
