@@ -581,7 +581,9 @@ __global__ void __launch_bounds__(kBlock) k_dv(FDev F, Consts<double> K, Consts<
     }
     const int nt = F.nt, n = F.n;
     const double *Q = F.qn;
-    double dti = K.local ? F.dt[i] : __longlong_as_double((long long)*F.dtslot);
+    // DT enters D only through VOL/DT: with LOCAL = 1 the record holds it, with
+    // LOCAL = 0 the record holds VOL and DT is the global minimum
+    const double dti = K.local ? 0.0 : __longlong_as_double((long long)*F.dtslot);
     if (!K.local) F.dt[i] = dti;
     int4 nbt = F.nct[i], nd4 = F.nod4[i];
     // raw[f]: neighbour id, with the BC type in the top bits for a ghost

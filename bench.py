@@ -55,6 +55,9 @@ def alg_bytes_per_cell(precision: str, m_over_n: float, local: int, kset: str = 
     update divides by are FP64.  Cell record: ANU, ANURT, VOL/DT, D2Q1..5 (8
     values); node record: UN, VN, TN, RHON, RN and the coordinates (7 values);
     frozen terms: D and VIS (9 values), or D + VIS combined (5, FP32-mixed).
+    FP64 k_dv forms the face vectors from the node coordinates it gathers for
+    the viscous faces (SC is not streamed) and, with LOCAL = 1, reads no DT
+    (the cell records carry VOL/DT).
     The merged and quad kernel sets have no k_dv launch (stage 1 forms the
     dissipation and viscous fluxes itself); the face set's k_vface (slot k_dv)
     evaluates each of the f_over_n faces per cell once: a 16-byte face record
@@ -69,7 +72,8 @@ def alg_bytes_per_cell(precision: str, m_over_n: float, local: int, kset: str = 
     pre = (q + dl + nct + g + (dt if local else 0) + rec
            + m_over_n * (4 + 8 * 4 + 5 * s))     # node: count word, 4 contributors
     base = dt + nct + geo + frozen             # every stage: DT, ids, geometry, D (+VIS)
-    dv = dt + nct + nod + q + rec + 10 * g + nodes + frozen
+    dv = ((0 if local else dt) + nct + nod + q + rec + (2 if precision == "fp64" else 10) * g
+          + nodes + frozen)
     st1 = q + base + 5 * s
     st23 = 5 * s + q + base + 5 * s
     st4 = 5 * s + q + base + q

@@ -193,6 +193,70 @@ def run_fd_case(name, mesh, flow, nrun, steps=(1e-6, 1e-5)):
     print(f"{name}: {nrun} iterations, J = {float(data['J_base']):.12e}")
 
 
+def converged_duct():
+    """The adjoint configuration's duct (193 x 65, 8 Hicks-Henne bumps at zero
+    amplitude) at CFL 0.5: from the cold start its flow settles after about
+    400,000 iterations and its L2 residual falls below 1e-8 of the first
+    iterations' near 700,000 (at the specified CFL 0.25 it keeps oscillating)."""
+    duct_kw = dict(nx=193, ny=65, jitter=0.01 * 193 / 385)
+    return gridgen.duct(bumps=[0.0] * 8, **duct_kw), gridgen.flow_duct(cfl=0.5), duct_kw
+
+
+def run_duct_fd_case(name, nrun, bumps=(3,), h_par=1e-6, h_bump=1e-3):
+    """The reference converged from the cold start on the converged_duct() case
+    at the base flow, at P0 and T0 moved by +-h_par relative and with the listed
+    bump amplitudes at +-h_bump mm: J = axial wall-pressure force + outlet mass
+    flow (tdt.objective.duct_objective) of each final state, for central
+    differences of the steady state's J; the base run's L2 history every 1000
+    iterations and its final state."""
+    import concurrent.futures as cf
+    from tdt import objective
+    mesh, flow, duct_kw = converged_duct()
+    runs = [("base", mesh, flow)]
+    for par in ("p0", "t0"):
+        for sg in (1, -1):
+            runs.append((f"{par}{'+' if sg > 0 else '-'}", mesh,
+                         dataclasses.replace(flow, **{par: getattr(flow, par) * (1 + sg * h_par)})))
+    for k in bumps:
+        for sg in (1, -1):
+            amps = [0.0] * 8
+            amps[k - 1] = sg * h_bump
+            runs.append((f"bump{k}{'+' if sg > 0 else '-'}", gridgen.duct(bumps=amps, **duct_kw),
+                         flow))
+
+    def one(item):
+        tag, m, f = item
+        work = tempfile.mkdtemp(prefix=f"tdt_golden_{name}_")
+        try:
+            f = dataclasses.replace(f, niter=nrun, istart=0)
+            caseio.write_case(work, f, m)
+            with open(os.path.join(work, "driver.in"), "w") as fh:
+                fh.write(f"{nrun} 0\n")
+            subprocess.check_call([os.path.join(REF, "ref_driver")], cwd=work,
+                                  stdout=subprocess.DEVNULL)
+            final = parse_final(os.path.join(work, "final.bin"))
+            l2, _ = parse_hist(os.path.join(work, "hist.bin"))
+            return tag, objective.duct_objective(m, f, final)[0], final, l2
+        finally:
+            shutil.rmtree(work)
+    with cf.ThreadPoolExecutor(max_workers=len(runs)) as ex:
+        res = list(ex.map(one, runs))
+    data = {"mesh_workload": np.array("duct_193x65"), "mesh_digest": np.array(mesh.digest())}
+    data["flow_json"] = np.array(json.dumps(dataclasses.asdict(flow)))
+    data["nrun"] = np.array(nrun)
+    data["h_par"] = np.array(h_par)
+    data["h_bump"] = np.array(h_bump)
+    data["bumps"] = np.array(bumps)
+    for tag, j, final, l2 in res:
+        data["J_" + tag] = np.array(j)
+        if tag == "base":
+            data["final"] = final
+            data["hist_l2_every1000"] = l2[999::1000]
+            data["hist_l2_head"] = l2[:50]
+    np.savez_compressed(os.path.join(OUT, name + ".npz"), **data)
+    print(f"{name}: {nrun} iterations, J = {float(data['J_base']):.12e}")
+
+
 def parse_tecplot(path, nods, neles):
     """Numbers of a tec_plot.dat: NODS x 8 node values, then NELES x 4 node ids
     (list-directed records may wrap, so the body is read as one token stream)."""
@@ -255,6 +319,9 @@ def main():
         run_case = lambda name, *a, **k: _rc(name, *a, **k) if name in only else None  # noqa: E731
         run_host_case = lambda name, *a, **k: _rh(name, *a, **k) if name in only else None  # noqa: E731
         run_fd_case = lambda name, *a, **k: _rf(name, *a, **k) if name in only else None  # noqa: E731
+        global run_duct_fd_case
+        _rd = run_duct_fd_case
+        run_duct_fd_case = lambda name, *a, **k: _rd(name, *a, **k) if name in only else None  # noqa: E731
     # 2-D channel: inlet 2, outlet 3 (extrapolated pressure), wall 5, symmetry 4
     m = gridgen.channel(nx=10, ny=6, length=50.0, height=10.0, h1=0.5, jitter=0.03,
                         bc={"left": 2, "right": 3, "bottom": 5, "top": 4}, seed=11)
@@ -308,6 +375,9 @@ def main():
     # fixed-point adjoint's check)
     m, f = converged_channel()
     run_fd_case("channel_48x16_converged", m, f, nrun=600000)
+    # the adjoint configuration's 193 x 65 duct converged (CFL 0.5): J = wall-pressure
+    # axial force + outlet mass flow at P0, T0 and bump 3 moved (about 3 hours)
+    run_duct_fd_case("duct_193x65_converged", nrun=850000)
     # the 97x49 channel / flat plate as specified: type-7 inlet with its inputs
     # defined (the executable cannot set them: no cross-check), its NITER = 2000
     m, f = gridgen.WORKLOADS["channel_97x49"][0](), gridgen.WORKLOADS["channel_97x49"][1]()

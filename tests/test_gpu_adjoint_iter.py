@@ -632,3 +632,89 @@ def test_fixed_point_adjoint_at_a_converged_nonuniform_flow(golden):
     for par in ("p0", "t0"):
         check(fd[par], adj[par], abs(fd[par]), 1e-4, par + " vs differences")
         check(ref[par], adj[par], abs(ref[par]), 1e-4, par + " vs reference differences")
+
+
+DUCT_CONVERGED = "duct_193x65_converged"
+DUCT_NRUN = 850000
+
+
+@pytest.fixture(scope="module")
+def converged_duct():
+    """The adjoint configuration's 193 x 65 duct (8 Hicks-Henne bumps at zero
+    amplitude; CFL 0.5, at which its flow settles: at the specified 0.25 it keeps
+    oscillating) run from the cold start for DUCT_NRUN iterations in one batch with
+    its variants at P0, T0 moved by +-1e-6 relative and bump 3 at +-1e-3 mm; then
+    the fixed-point adjoint of J = axial wall-pressure force + outlet mass flow at
+    the base case's steady state, with the node gradient chained through the
+    generator's dX/da_3."""
+    import dataclasses as dc
+    import tdt
+    from tdt import gridgen, objective
+    duct_kw = dict(nx=193, ny=65, jitter=0.01 * 193 / 385)
+    mesh, flow = gridgen.duct(bumps=[0.0] * 8, **duct_kw), gridgen.flow_duct(cfl=0.5)
+    cases, tags = [(mesh, flow)], ["base"]
+    for par in ("p0", "t0"):
+        for sg in (1.0, -1.0):
+            cases.append((mesh, dc.replace(flow, **{par: getattr(flow, par) * (1.0 + sg * 1e-6)})))
+            tags.append(par + ("+" if sg > 0 else "-"))
+    for sg in (1.0, -1.0):
+        amps = [0.0] * 8
+        amps[2] = sg * 1e-3
+        cases.append((gridgen.duct(bumps=amps, **duct_kw), flow))
+        tags.append("bump3" + ("+" if sg > 0 else "-"))
+    with tdt.Batch(cases) as b:
+        b.cold_start()
+        l2a, _, bad = b.iterate(50)
+        assert bad == 0
+        b.iterate(DUCT_NRUN - 51, history=False)
+        l2b, _, bad = b.iterate(1)
+        assert bad == 0
+        drop = (l2b[-1] / l2a.max(axis=0)).max(axis=1)
+        states = [b.get_state(k) for k in range(len(cases))]
+    J = {t: objective.duct_objective(m, f, c)[0] for t, (m, f), c in zip(tags, cases, states)}
+    fd = {par: (J[par + "+"] - J[par + "-"]) / (2e-6 * getattr(flow, par)) for par in ("p0", "t0")}
+    fd["bump3"] = (J["bump3+"] - J["bump3-"]) / 2e-3
+    j, djdq, djdxy = objective.duct_objective(mesh, flow, states[0])
+    with tdt.Solver(mesh, flow) as s:
+        rinit = s.cold_start()
+        _, gp, gxy, iters, est = s.adjoint_fixed_point(states[0], rinit, djdq, max_iter=2000000,
+                                                       tol=1e-9, nodes=True)
+    # RINIT = AMUINIT / RHOINIT of the cold start: proportional to T0^1.6 / P0
+    adj = {"p0": gp[0] - gp[2] * rinit / flow.p0, "t0": gp[1] + gp[2] * 1.6 * rinit / flow.t0,
+           "bump3": float(np.sum((gxy + djdxy) * objective.bump_node_derivatives(8, **duct_kw)[2]))}
+    return dict(flow=flow, drop=drop, J=j, fd=fd, adj=adj, iters=iters, est=est, state=states[0])
+
+
+def test_fixed_point_adjoint_on_the_converged_adjoint_duct(converged_duct):
+    """The 193 x 65 duct converged to an L2 residual drop below 1e-8 (every case of
+    the batch); the fixed-point adjoint's dJ/dP0, dJ/dT0 and dJ/d(bump 3) of the
+    steady state against central differences of the converged runs, within 1e-4."""
+    r = converged_duct
+    assert np.all(r["drop"] <= 1e-8), r["drop"]
+    assert r["est"] <= 1e-9, (r["iters"], r["est"])
+    print(f"duct converged: L2 drop {r['drop'].max():.2e}, J = {r['J']:.12e}; adjoint "
+          f"{r['iters']} iterations (error estimate {r['est']:.1e}); adjoint {r['adj']}, "
+          f"differences {r['fd']}")
+    for key in ("p0", "t0", "bump3"):
+        check(r["fd"][key], r["adj"][key], abs(r["fd"][key]), 1e-4, key + " vs differences")
+
+
+@pytest.mark.skipif(not __import__("os").path.exists(__import__("os").path.join(
+    REPO, "tests", "golden", DUCT_CONVERGED + ".npz")), reason=DUCT_CONVERGED + " fixture not generated")
+def test_converged_adjoint_duct_against_the_reference(converged_duct, golden):
+    """The reference solver run on the same duct for the same iterations
+    (tests/golden): the converged J agrees to 1e-9, and the adjoint's gradients
+    agree with the reference's central differences (P0, T0 by +-1e-6 relative,
+    bump 3 by +-1e-3 mm) within 1e-4."""
+    r = converged_duct
+    g = golden(DUCT_CONVERGED)
+    assert int(g.nrun) == DUCT_NRUN
+    jb = float(g["J_base"])
+    assert abs(r["J"] - jb) <= 1e-9 * abs(jb), (r["J"], jb)
+    h, hb = float(g["h_par"]), float(g["h_bump"])
+    ref = {par: (float(g[f"J_{par}+"]) - float(g[f"J_{par}-"])) / (2 * h * getattr(r["flow"], par))
+           for par in ("p0", "t0")}
+    ref["bump3"] = (float(g["J_bump3+"]) - float(g["J_bump3-"])) / (2 * hb)
+    print(f"reference differences {ref}, adjoint {r['adj']}")
+    for key in ("p0", "t0", "bump3"):
+        check(ref[key], r["adj"][key], abs(ref[key]), 1e-4, key + " vs reference differences")
