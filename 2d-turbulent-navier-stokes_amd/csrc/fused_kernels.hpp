@@ -771,12 +771,15 @@ __global__ void __launch_bounds__(kBlock) k_rk(FDev F, Consts<double> K, Consts<
                 dissipation_rec<R, SD, true>(F, i, nbr, ci, K.local, dti, dsum, qf);
             else
                 dissipation_rec<R, SD>(F, i, nbr, ci, K.local, dti, dsum);
-            // stored now in either form (FP32-mixed overwrites it with D + VIS
-            // after the faces): the store keeps the face loop's loads after it
+            // FP64 stores D now.  FP32-mixed keeps it in registers and stores
+            // D + VIS after the faces: its store here never runs (n >= 1), but
+            // as a store it keeps the face loop's loads after this point, which
+            // holds the register count down, and it writes nothing twice
 #pragma unroll
-            for (int q = 0; q < 5; ++q) {
-                dd[q] = (double)(SD)dsum[q];
-                stv<SD>(F.dd, (size_t)q * n + i, dd[q]);
+            for (int q = 0; q < 5; ++q) dd[q] = (double)(SD)dsum[q];
+            if (!kCombinedDV<SD> || n < 0) {
+#pragma unroll
+                for (int q = 0; q < 5; ++q) stv<SD>(F.dd, (size_t)q * n + i, dd[q]);
             }
         } else {
             // D and VIS frozen at stage 1 (FP32-mixed: combined in D)
