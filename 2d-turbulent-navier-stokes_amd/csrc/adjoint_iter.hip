@@ -713,6 +713,11 @@ __global__ void __launch_bounds__(kBlock) k_add(int count, const double *x, doub
     if (i < count) y[i] = y[i] + x[i];
 }
 
+__global__ void __launch_bounds__(kBlock) k_zero(int count, double *p) {
+    const int i = gid();
+    if (i < count) p[i] = 0.0;
+}
+
 // cells' [comp][nt] rows -> [comp][n] (and back)
 __global__ void __launch_bounds__(kBlock) k_pack(int n, int nt, const double *src, double *dst) {
     const int i = gid();
@@ -791,6 +796,11 @@ hipError_t launch_stage_vjp(tdt_ctx *c, double alpha, const double *q, const dou
 
 hipError_t launch_add(tdt_ctx *c, int count, const double *x, double *y) {
     hipLaunchKernelGGL(k_add, dim3(nblocks(count)), dim3(kBlock), 0, c->stream, count, x, y);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero(tdt_ctx *c, int count, double *p) {
+    hipLaunchKernelGGL(k_zero, dim3(nblocks(count)), dim3(kBlock), 0, c->stream, count, p);
     return hipGetLastError();
 }
 

@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <memory>
 #include <string>
@@ -256,9 +257,10 @@ int frozen_vjp(tdt_ctx *c, const double *qn) {
     const tdt::Dev &D = c->D;
     const int n = D.n;
     const size_t n5 = (size_t)5 * n;
-    HIPCALL(c, zero(c, b.lam, n5), "adjoint: clear");
-    HIPCALL(c, zero(c, b.W, n5), "adjoint: clear");
-    HIPCALL(c, zero(c, b.gdt, n), "adjoint: clear");
+    // cleared by kernels: the fixed-point loop captures this in a graph
+    HIPCALL(c, tdt::launch_zero(c, (int)n5, b.lam), "adjoint: clear");
+    HIPCALL(c, tdt::launch_zero(c, (int)n5, b.W), "adjoint: clear");
+    HIPCALL(c, tdt::launch_zero(c, n, b.gdt), "adjoint: clear");
     // b.gvol and b.grtol collect the stage reverses' volume and RTOL terms, which
     // nothing here reads (iteration_vjp_core clears them before it uses them);
     // the residual and operator reverses skip their parameter sums (gp nullptr)
@@ -455,6 +457,47 @@ int tdt_iteration_vjp(tdt_ctx *c, const double *cst, double rinit, const double 
     return TDT_OK;
 }
 
+namespace {
+
+// The iterations between two convergence checks of the fixed-point loop, captured as
+// one graph: each is frozen_vjp (b.mu -> b.lam) plus dJ/dQ, then the two buffers swap
+// roles.  An even count leaves the roles where they started, so one graph per role
+// assignment serves every span.  The loop only launches kernels (frozen_vjp clears
+// its buffers with kernels, not hipMemsetAsync).
+struct SpanGraphs {
+    hipGraphExec_t exec[2] = {nullptr, nullptr};   // by which buffer holds mu
+    ~SpanGraphs() {
+        for (hipGraphExec_t e : exec)
+            if (e) (void)hipGraphExecDestroy(e);
+    }
+};
+
+int capture_span(tdt_ctx *c, const double *qn, int span, hipGraphExec_t *exec) {
+    tdt::IterAdjBufs &b = *c->iadj;
+    const int n5 = 5 * c->pm.n;
+    if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        c->err = "fixed-point adjoint: graph capture failed";
+        return TDT_ERR_DEVICE;
+    }
+    int rc = TDT_OK;
+    for (int k = 0; k < span && !rc; ++k) {
+        rc = frozen_vjp(c, qn);
+        if (!rc && tdt::launch_add(c, n5, b.gJ, b.lam) != hipSuccess) rc = TDT_ERR_DEVICE;
+        std::swap(b.mu, b.lam);
+    }
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (!rc && e == hipSuccess) e = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+    if (g) (void)hipGraphDestroy(g);
+    if (!rc && e != hipSuccess) {
+        c->err = std::string("fixed-point adjoint: graph: ") + hipGetErrorString(e);
+        rc = TDT_ERR_DEVICE;
+    }
+    return rc;
+}
+
+}  // namespace
+
 int tdt_adjoint_fixed_point(tdt_ctx *c, const double *cst, double rinit, const double *dj_dc,
                             int32_t max_iter, double tol, double *lam, double *gparam,
                             double *gxy, int32_t *iters, double *change) {
@@ -483,11 +526,29 @@ int tdt_adjoint_fixed_point(tdt_ctx *c, const double *cst, double rinit, const d
     // error is about d_k rho / (1 - rho): rho is estimated from two successive
     // checks, rho^kCheck = d_k / d_{k-kCheck}, and the iteration stops when that
     // estimate is <= tol (d_k alone underestimates it when rho is near 1).
-    constexpr int kCheck = 25;
+    // The kCheck - 1 iterations before each checked one run as a captured graph
+    // (TDT_FIXED_POINT_EAGER set: launched one by one, the same kernels in the same
+    // order, for comparison).
+    constexpr int kCheck = 25, kSpan = kCheck - 1;
+    static_assert(kSpan % 2 == 0, "a span must leave the buffers' roles unchanged");
+    const bool eager = std::getenv("TDT_FIXED_POINT_EAGER") != nullptr;
+    SpanGraphs spans;
+    double *const mu_first = b.mu;
     std::vector<double> h0(n5), h1(n5);
     int it = 0;
     double rel = INFINITY, prev = INFINITY;
     while (!rc && it < max_iter && !(rel <= tol)) {
+        if (!eager && it % kCheck == 0 && it + kCheck <= max_iter) {
+            const int role = b.mu == mu_first ? 0 : 1;
+            if (!spans.exec[role]) rc = capture_span(c, b.qn, kSpan, &spans.exec[role]);
+            if (rc) break;
+            if (hipGraphLaunch(spans.exec[role], c->stream) != hipSuccess) {
+                c->err = "fixed-point adjoint: graph launch failed";
+                rc = TDT_ERR_DEVICE;
+                break;
+            }
+            it += kSpan;
+        }
         const bool check = (it + 1) % kCheck == 0 || it + 1 == max_iter;
         if (check)   // lam_k
             HIPCALL(c, hipMemcpyAsync(h0.data(), b.mu, n5 * sizeof(double),

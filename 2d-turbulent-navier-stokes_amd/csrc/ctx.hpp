@@ -189,6 +189,8 @@ hipError_t launch_stage_vjp(tdt_ctx *c, double alpha, const double *q, const dou
                             const double *vol, const double *mu, double *lam, double *wr,
                             double *W, double *gdt, double *gvol, double *grtol);
 hipError_t launch_add(tdt_ctx *c, int count, const double *x, double *y);
+// p[0..count) = 0 by a kernel (unlike hipMemsetAsync, safe inside a captured graph)
+hipError_t launch_zero(tdt_ctx *c, int count, double *p);
 // the cells' C rows [5][nt] -> [5][n]; and [5][n] -> C and COLD
 hipError_t launch_pack(tdt_ctx *c, const double *src, double *dst);
 hipError_t launch_unpack(tdt_ctx *c, const double *src);

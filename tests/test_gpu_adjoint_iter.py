@@ -532,6 +532,32 @@ def steady_outlet_mass_flow(mesh, flow, p0, t0):
     return float(rho * u * np.sum(sx))
 
 
+def test_fixed_point_graph_spans_equal_eager_iterations(monkeypatch):
+    """The fixed-point loop runs the iterations between its convergence checks as a
+    captured graph, replayed many times with eager iterations and host reads in
+    between; launched one by one (TDT_FIXED_POINT_EAGER) the same kernels give the
+    same adjoint, parameter and node gradients, iteration count and error estimate,
+    bit for bit.  Odd max_iter: the run ends in a partial span, eagerly."""
+    import tdt
+    from tdt import objective
+    mesh, flow = uniform_channel()
+    out = {}
+    for mode in ("graph", "eager"):
+        if mode == "eager":
+            monkeypatch.setenv("TDT_FIXED_POINT_EAGER", "1")
+        with tdt.Solver(mesh, flow) as s:
+            rinit = s.cold_start()
+            c = s.get_state()
+            o = objective.outlet_faces(mesh)
+            sx, _, _ = objective._scx(mesh, 0, o[:, 1], o[:, 2])
+            dj = np.zeros_like(c)
+            np.add.at(dj[1], o[:, 0], sx)
+            out[mode] = s.adjoint_fixed_point(c, rinit, dj, max_iter=1237, tol=0.0, nodes=True)
+    (lg, gg, xg, ig, cg), (le, ge, xe, ie, ce) = out["graph"], out["eager"]
+    assert ig == ie and cg == ce
+    assert np.array_equal(lg, le) and np.array_equal(gg, ge) and np.array_equal(xg, xe)
+
+
 def test_fixed_point_adjoint_at_a_steady_state():
     """The adjoint iterated to its fixed point, lam = Phi_Q^T lam + dJ/dQ, at a
     steady state of the iteration (the uniform channel): J = outlet mass flow;
