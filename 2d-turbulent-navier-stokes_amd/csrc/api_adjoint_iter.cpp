@@ -464,15 +464,21 @@ namespace {
 // roles.  An even count leaves the roles where they started, so one graph per role
 // assignment serves every span.  The loop only launches kernels (frozen_vjp clears
 // its buffers with kernels, not hipMemsetAsync).
+// Each graph is kept, like its executable, until the loop ends (as the fused path
+// keeps its graphs).
 struct SpanGraphs {
+    hipGraph_t graph[2] = {nullptr, nullptr};
     hipGraphExec_t exec[2] = {nullptr, nullptr};   // by which buffer holds mu
     ~SpanGraphs() {
-        for (hipGraphExec_t e : exec)
-            if (e) (void)hipGraphExecDestroy(e);
+        for (int r = 0; r < 2; ++r) {
+            if (exec[r]) (void)hipGraphExecDestroy(exec[r]);
+            if (graph[r]) (void)hipGraphDestroy(graph[r]);
+        }
     }
 };
 
-int capture_span(tdt_ctx *c, const double *qn, int span, hipGraphExec_t *exec) {
+int capture_span(tdt_ctx *c, const double *qn, int span, hipGraph_t *graph,
+                 hipGraphExec_t *exec) {
     tdt::IterAdjBufs &b = *c->iadj;
     const int n5 = 5 * c->pm.n;
     if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
@@ -485,10 +491,8 @@ int capture_span(tdt_ctx *c, const double *qn, int span, hipGraphExec_t *exec) {
         if (!rc && tdt::launch_add(c, n5, b.gJ, b.lam) != hipSuccess) rc = TDT_ERR_DEVICE;
         std::swap(b.mu, b.lam);
     }
-    hipGraph_t g = nullptr;
-    hipError_t e = hipStreamEndCapture(c->stream, &g);
-    if (!rc && e == hipSuccess) e = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
-    if (g) (void)hipGraphDestroy(g);
+    hipError_t e = hipStreamEndCapture(c->stream, graph);
+    if (!rc && e == hipSuccess) e = hipGraphInstantiate(exec, *graph, nullptr, nullptr, 0);
     if (!rc && e != hipSuccess) {
         c->err = std::string("fixed-point adjoint: graph: ") + hipGetErrorString(e);
         rc = TDT_ERR_DEVICE;
@@ -540,7 +544,8 @@ int tdt_adjoint_fixed_point(tdt_ctx *c, const double *cst, double rinit, const d
     while (!rc && it < max_iter && !(rel <= tol)) {
         if (!eager && it % kCheck == 0 && it + kCheck <= max_iter) {
             const int role = b.mu == mu_first ? 0 : 1;
-            if (!spans.exec[role]) rc = capture_span(c, b.qn, kSpan, &spans.exec[role]);
+            if (!spans.exec[role])
+                rc = capture_span(c, b.qn, kSpan, &spans.graph[role], &spans.exec[role]);
             if (rc) break;
             if (hipGraphLaunch(spans.exec[role], c->stream) != hipSuccess) {
                 c->err = "fixed-point adjoint: graph launch failed";
