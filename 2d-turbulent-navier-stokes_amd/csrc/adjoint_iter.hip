@@ -718,6 +718,23 @@ __global__ void __launch_bounds__(kBlock) k_zero(int count, double *p) {
     if (i < count) p[i] = 0.0;
 }
 
+// the fixed-point iteration's two clears of [5][n] values and one of [n], in one launch
+__global__ void __launch_bounds__(kBlock) k_zero_5n_5n_n(int n, double *a, double *b, double *c) {
+    const int i = gid();
+    if (i < 5 * n) {
+        a[i] = 0.0;
+        b[i] = 0.0;
+    }
+    if (i < n) c[i] = 0.0;
+}
+
+// y = ((y + x1) + x2) + x3: three k_add launches in one, the same sums in the same order
+__global__ void __launch_bounds__(kBlock) k_add3(int count, const double *x1, const double *x2,
+                                                 const double *x3, double *y) {
+    const int i = gid();
+    if (i < count) y[i] = y[i] + x1[i] + x2[i] + x3[i];
+}
+
 // cells' [comp][nt] rows -> [comp][n] (and back)
 __global__ void __launch_bounds__(kBlock) k_pack(int n, int nt, const double *src, double *dst) {
     const int i = gid();
@@ -801,6 +818,20 @@ hipError_t launch_add(tdt_ctx *c, int count, const double *x, double *y) {
 
 hipError_t launch_zero(tdt_ctx *c, int count, double *p) {
     hipLaunchKernelGGL(k_zero, dim3(nblocks(count)), dim3(kBlock), 0, c->stream, count, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_5n_5n_n(tdt_ctx *c, double *a, double *b, double *d) {
+    const int n = c->D.n;
+    hipLaunchKernelGGL(k_zero_5n_5n_n, dim3(nblocks(5 * n)), dim3(kBlock), 0, c->stream, n, a, b,
+                       d);
+    return hipGetLastError();
+}
+
+hipError_t launch_add3(tdt_ctx *c, int count, const double *x1, const double *x2,
+                       const double *x3, double *y) {
+    hipLaunchKernelGGL(k_add3, dim3(nblocks(count)), dim3(kBlock), 0, c->stream, count, x1, x2,
+                       x3, y);
     return hipGetLastError();
 }
 

@@ -249,18 +249,16 @@ int fixed_point_prepare(tdt_ctx *c, const double *qn) {
     return TDT_OK;
 }
 
-// b.mu -> b.lam = Phi_Q^T mu with the blocks fixed_point_prepare kept (no
-// parameter or node derivatives)
-int frozen_vjp(tdt_ctx *c, const double *qn) {
+// b.mu -> b.lam = Phi_Q^T mu + gj with the blocks fixed_point_prepare kept (no
+// parameter or node derivatives): one fixed-point iteration
+int frozen_vjp(tdt_ctx *c, const double *qn, const double *gj) {
     tdt::IterAdjBufs &b = *c->iadj;
     tdt::AdjBufs &a = *c->adj;
     const tdt::Dev &D = c->D;
     const int n = D.n;
     const size_t n5 = (size_t)5 * n;
-    // cleared by kernels: the fixed-point loop captures this in a graph
-    HIPCALL(c, tdt::launch_zero(c, (int)n5, b.lam), "adjoint: clear");
-    HIPCALL(c, tdt::launch_zero(c, (int)n5, b.W), "adjoint: clear");
-    HIPCALL(c, tdt::launch_zero(c, n, b.gdt), "adjoint: clear");
+    // cleared by a kernel: the fixed-point loop captures this in a graph
+    HIPCALL(c, tdt::launch_zero_5n_5n_n(c, b.lam, b.W, b.gdt), "adjoint: clear");
     // b.gvol and b.grtol collect the stage reverses' volume and RTOL terms, which
     // nothing here reads (iteration_vjp_core clears them before it uses them);
     // the residual and operator reverses skip their parameter sums (gp nullptr)
@@ -273,13 +271,13 @@ int frozen_vjp(tdt_ctx *c, const double *qn) {
                                          a.g, nullptr), "adjoint: residual");
         if (k > 0)
             mu = a.g;   // the stream orders the next stage's read before a.g is rewritten
-        else
-            HIPCALL(c, tdt::launch_add(c, (int)n5, a.g, b.lam), "adjoint: accumulate");
     }
     HIPCALL(c, tdt::launch_op_vjp(c, b.JE, b.JEp, nullptr, b.W, b.gdt, b.gvol, b.grtol, b.tptr,
                                   b.tlist, a.nptr, a.nlist, b.part, b.g, nullptr, nullptr),
             "adjoint: operators");
-    HIPCALL(c, tdt::launch_add(c, (int)n5, b.g, b.lam), "adjoint: accumulate");
+    // lam += stage 1's residual adjoint (a.g, which the operators leave alone), the
+    // operators' (b.g) and gj, in that order
+    HIPCALL(c, tdt::launch_add3(c, (int)n5, a.g, b.g, gj, b.lam), "adjoint: accumulate");
     return TDT_OK;
 }
 
@@ -460,7 +458,7 @@ int tdt_iteration_vjp(tdt_ctx *c, const double *cst, double rinit, const double 
 namespace {
 
 // The iterations between two convergence checks of the fixed-point loop, captured as
-// one graph: each is frozen_vjp (b.mu -> b.lam) plus dJ/dQ, then the two buffers swap
+// one graph: each is frozen_vjp (b.mu -> b.lam, dJ/dQ added), then the two buffers swap
 // roles.  An even count leaves the roles where they started, so one graph per role
 // assignment serves every span.  The loop only launches kernels (frozen_vjp clears
 // its buffers with kernels, not hipMemsetAsync).
@@ -480,15 +478,13 @@ struct SpanGraphs {
 int capture_span(tdt_ctx *c, const double *qn, int span, hipGraph_t *graph,
                  hipGraphExec_t *exec) {
     tdt::IterAdjBufs &b = *c->iadj;
-    const int n5 = 5 * c->pm.n;
     if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
         c->err = "fixed-point adjoint: graph capture failed";
         return TDT_ERR_DEVICE;
     }
     int rc = TDT_OK;
     for (int k = 0; k < span && !rc; ++k) {
-        rc = frozen_vjp(c, qn);
-        if (!rc && tdt::launch_add(c, n5, b.gJ, b.lam) != hipSuccess) rc = TDT_ERR_DEVICE;
+        rc = frozen_vjp(c, qn, b.gJ);
         std::swap(b.mu, b.lam);
     }
     hipError_t e = hipStreamEndCapture(c->stream, graph);
@@ -558,9 +554,8 @@ int tdt_adjoint_fixed_point(tdt_ctx *c, const double *cst, double rinit, const d
         if (check)   // lam_k
             HIPCALL(c, hipMemcpyAsync(h0.data(), b.mu, n5 * sizeof(double),
                                       hipMemcpyDeviceToHost, c->stream), "download");
-        rc = frozen_vjp(c, b.qn);
+        rc = frozen_vjp(c, b.qn, b.gJ);
         if (rc) break;
-        HIPCALL(c, tdt::launch_add(c, (int)n5, b.gJ, b.lam), "adjoint: accumulate");
         ++it;
         if (check) {
             HIPCALL(c, hipMemcpyAsync(h1.data(), b.lam, n5 * sizeof(double),

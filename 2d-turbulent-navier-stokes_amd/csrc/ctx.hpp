@@ -189,8 +189,13 @@ hipError_t launch_stage_vjp(tdt_ctx *c, double alpha, const double *q, const dou
                             const double *vol, const double *mu, double *lam, double *wr,
                             double *W, double *gdt, double *gvol, double *grtol);
 hipError_t launch_add(tdt_ctx *c, int count, const double *x, double *y);
-// p[0..count) = 0 by a kernel (unlike hipMemsetAsync, safe inside a captured graph)
+// p[0..count) = 0 by a kernel (unlike hipMemsetAsync, safe inside a captured graph);
+// a[0..5n) = b[0..5n) = d[0..n) = 0 in one launch
 hipError_t launch_zero(tdt_ctx *c, int count, double *p);
+hipError_t launch_zero_5n_5n_n(tdt_ctx *c, double *a, double *b, double *d);
+// y += x1, then += x2, then += x3 (one launch, the sums of three launch_add calls)
+hipError_t launch_add3(tdt_ctx *c, int count, const double *x1, const double *x2,
+                       const double *x3, double *y);
 // the cells' C rows [5][nt] -> [5][n]; and [5][n] -> C and COLD
 hipError_t launch_pack(tdt_ctx *c, const double *src, double *dst);
 hipError_t launch_unpack(tdt_ctx *c, const double *src);
