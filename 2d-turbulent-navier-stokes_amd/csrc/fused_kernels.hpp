@@ -392,23 +392,32 @@ __device__ __forceinline__ void node_gather_list(const FDev &F, const Consts<R> 
 // ---------------------------------------------------------------------------------
 // k_pre: ncb8 cell blocks and nnb8 node blocks (both padded to multiples of 8
 // and spread so that each XCD works on contiguous cells and the nodes around
-// them, alternating between the two), and a last block of its own that folds
-// the previous iteration's monitor partials (off every cell's path).  Cells:
+// them, alternating between the two), after a first block of its own that folds
+// the previous iteration's monitor partials (off every cell's path; one per case
+// in a batch).  Cells:
 // primitives of Q^n, DT, and the dissipation record (sensors, VOL/DT, D2Q);
 // nodes: the nodal averages.
 template <class SD, class R, bool B6, bool BT>
 __global__ void __launch_bounds__(kBlock) k_pre(FDev F, Consts<double> K, Consts<R> KR,
                                                 int nb_partials, int ncb8, int nnb8) {
+    // The folding blocks (one per case of a batch) come first: dispatched with the
+    // first blocks, they run beside the cell and node blocks, where as the last
+    // blocks their fold outlasted them (FP64 ramp: 73 -> 64 us).  Their count is
+    // padded to a multiple of 8, which keeps every other block on the XCD that the
+    // mapping below assumes (blocks are dispatched round-robin over the 8 XCDs).
     int b = blockIdx.x;
-    if constexpr (BT) {          // one folding block per case
-        const int first_fold = (int)gridDim.x - F.ncases;
-        if (b >= first_fold) {
-            fold_case(F, b - first_fold);
+    {
+        const int nfold = BT ? F.ncases : 1, nfold8 = (nfold + 7) & ~7;
+        if (b < nfold8) {
+            if (b < nfold) {
+                if constexpr (BT)
+                    fold_case(F, b);
+                else
+                    fold_monitor(F, nb_partials);
+            }
             return;
         }
-    } else if (b == (int)gridDim.x - 1) {
-        fold_monitor(F, nb_partials);
-        return;
+        b -= nfold8;
     }
     // Each XCD takes its eighth of the cell blocks and of the node blocks,
     // alternating between them, so that a node block runs while the states of
@@ -1030,8 +1039,8 @@ hipError_t enqueue_iteration(tdt_ctx *c, FusedBufs &B, const Consts<R> &KR) {
                nparts);
     } else {
         const int nnb8 = (B.nb_nodes + 7) & ~7;
-        const unsigned nfold = BT ? F.ncases : 1;
-        launch(c, B, 0, k_pre<SD, R, B6, BT>, F.ncb8 + nnb8 + nfold, F, c->K, KR, nparts,
+        const unsigned nfold8 = ((BT ? F.ncases : 1) + 7) & ~7;   // the folding blocks, padded
+        launch(c, B, 0, k_pre<SD, R, B6, BT>, nfold8 + F.ncb8 + nnb8, F, c->K, KR, nparts,
                F.ncb8, nnb8);
     }
     switch (ks & kSt1Mask) {
