@@ -214,11 +214,13 @@ def converged_comparison(mesh, flow, iters):
 REF_BUILDS = (("turbnstwod", 60000), ("turbnstwod_big", 1060000))   # (executable, its NN)
 
 
-def cpu_baseline(mesh, flow, target_s: float = 15.0):
+def cpu_baseline(mesh, flow, target_s: float = 15.0, iters: int = 0):
     """The reference executable (built from its sources by oracle/build_ref.sh,
     the smallest build whose NN holds the grid's cells + ghosts) on this host,
-    one core.  The case is written once; loop time = t(NITER=k) - t(NITER=0),
-    so reading the case and writing the results cancel out."""
+    one core.  The case is written once; loop time = t(NITER=k0+k) - t(NITER=k0),
+    so reading the case, writing the results and the first k0 iterations (which
+    also fault in the statically sized arrays) cancel out.  k is sized to about
+    target_s seconds, or fixed by iters."""
     ref = None
     for name, nn in REF_BUILDS:
         path = os.path.join(REPO, "oracle", "_ref", name)
@@ -239,18 +241,16 @@ def cpu_baseline(mesh, flow, target_s: float = 15.0):
                            cwd=work, stdout=subprocess.DEVNULL, check=True)
             return time.perf_counter() - t0
         t_0 = run(0)
-        k = 5 if mesh.neles < 200000 else 8     # about 1 s per iteration at 1 M cells
-        loop = run(k) - t_0
-        if loop < 0.5 * target_s:
-            k = int(max(k + 1, min(2000, k * target_s / max(loop, 1e-6))))
-            loop = run(k) - t_0
-        loop = max(loop, 1e-9)
+        k0 = 5 if mesh.neles < 200000 else 8    # about 1 s per iteration at 1 M cells
+        t_k0 = run(k0)
+        k = iters or int(max(k0, min(2000, target_s * k0 / max(t_k0 - t_0, 1e-6))))
+        loop = max(run(k0 + k) - t_k0, 1e-9)
         value = mesh.neles * k / loop / 1e6
         return {"value": round(value, 4), "unit": "Mcell-iters/s", "cores": 1,
                 "kind": "reference",
                 "sample": f"reference {os.path.basename(ref)} (FP64, amdflang -O3), same grid "
-                          f"and flow.in, {k} iterations, loop time = t(NITER={k}) - "
-                          f"t(NITER=0) = {loop:.2f} s"}
+                          f"and flow.in, {k} iterations, loop time = t(NITER={k0 + k}) - "
+                          f"t(NITER={k0}) = {loop:.2f} s"}
     finally:
         shutil.rmtree(work, ignore_errors=True)
 
@@ -717,6 +717,9 @@ def main():
     ap.add_argument("--cpu-sweep", type=int, default=0, metavar="NCASES", help=argparse.SUPPRESS)
     ap.add_argument("--cpu-iters", type=int, default=20, help=argparse.SUPPRESS)
     ap.add_argument("--cpu-cores", type=int, default=0, help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-only", type=int, default=0, metavar="NITER",
+                    help="time the reference executable alone over NITER iterations of the "
+                         "workload on one host core, print its JSON line and exit")
     ap.add_argument("--adjoint", type=int, default=0, metavar="NITER",
                     help="run the 193x65 duct adjoint study over NITER iterations, print its "
                          "JSON line and exit")
@@ -757,6 +760,12 @@ def main():
         avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
         cores = args.cpu_cores or min(16, avail or 1)
         print(json.dumps(cpu_sweep(args.cpu_sweep, args.cpu_iters, cores)), flush=True)
+        return
+    if args.cpu_only:    # never touches the GPU
+        mk_mesh, mk_flow = gridgen.WORKLOADS[args.workload]
+        print(json.dumps({"workload": args.workload,
+                          "cpu_baseline": cpu_baseline(mk_mesh(), mk_flow(),
+                                                       iters=args.cpu_only)}), flush=True)
         return
     if args.adjoint:
         out = adjoint_study(args.adjoint, ref_iters=args.adjoint_ref_iters,
