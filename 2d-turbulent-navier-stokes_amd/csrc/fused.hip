@@ -186,6 +186,8 @@ int fused_setup(tdt_ctx *c) {
         }
     }
     if (c->batch) B->kset &= kSt1Split | kStPreload;   // no quad form for a batch
+    // the wave node tables: FP64 single cases with LOCAL = 1 only
+    if (c->precision != TDT_FP64 || c->batch || !c->K.local) B->kset &= ~kDvNodeLds;
     // fused-only neighbour and node-gather tables (built from the prepared mesh)
     const PreparedMesh &pm = c->pm;
     if (D.nt > kIdMask) {
@@ -266,6 +268,51 @@ int fused_setup(tdt_ctx *c) {
                  hipSuccess;
         F.vfl = d_vfl;
         F.cvf = d_cvf;
+    }
+    if (c->precision == TDT_FP64 && !c->batch && ok) {
+        // node tables of the wavefronts (kDvNodeLds): the distinct nodes of each
+        // wave's 64 cells, ascending, and each cell's four slots in its wave's list
+        const int nwv = B->nb_cells * (kBlock / kWave);
+        auto wave_nodes = [&](int w, std::vector<int> &v) {
+            v.clear();
+            for (int i = w * kWave; i < std::min(D.n, (w + 1) * kWave); ++i)
+                for (int q = 0; q < 4; ++q) v.push_back(pm.nod4[4 * i + q]);
+            std::sort(v.begin(), v.end());
+            v.erase(std::unique(v.begin(), v.end()), v.end());
+            if (v.empty()) v.push_back(0);   // a wave past the last cell: any node
+        };
+        std::vector<int> v;
+        size_t cap = 0;
+        for (int w = 0; w < nwv; ++w) {
+            wave_nodes(w, v);
+            cap = std::max(cap, v.size());
+        }
+        cap = (cap + kWave - 1) / kWave * kWave;   // whole 64-lane copy rounds, <= 256
+        std::vector<int> bnode((size_t)nwv * cap), nloc((size_t)2 * D.n);
+        for (int w = 0; w < nwv; ++w) {
+            wave_nodes(w, v);
+            std::copy(v.begin(), v.end(), bnode.begin() + (size_t)w * cap);
+            std::fill(bnode.begin() + (size_t)w * cap + v.size(),
+                      bnode.begin() + (size_t)(w + 1) * cap, v.back());
+            for (int i = w * kWave; i < std::min(D.n, (w + 1) * kWave); ++i) {
+                int sl[4];
+                for (int q = 0; q < 4; ++q)
+                    sl[q] = (int)(std::lower_bound(v.begin(), v.end(), pm.nod4[4 * i + q]) -
+                                  v.begin());
+                nloc[2 * i] = sl[0] | sl[1] << 16;
+                nloc[2 * i + 1] = sl[2] | sl[3] << 16;
+            }
+        }
+        int *d_bnode = dalloc<int>(*B, bnode.size());
+        int *d_nloc = dalloc<int>(*B, nloc.size());
+        ok = d_bnode && d_nloc &&
+             hipMemcpy(d_bnode, bnode.data(), bnode.size() * sizeof(int),
+                       hipMemcpyHostToDevice) == hipSuccess &&
+             hipMemcpy(d_nloc, nloc.data(), nloc.size() * sizeof(int), hipMemcpyHostToDevice) ==
+                 hipSuccess;
+        F.bnode = d_bnode;
+        F.nloc = reinterpret_cast<const int2 *>(d_nloc);
+        F.nl_cap = (int)cap;
     }
     int *d_nct = dalloc<int>(*B, nct.size());
     int *d_ncon = dalloc<int>(*B, ncon.size());
@@ -417,14 +464,24 @@ static hipError_t autotune(tdt_ctx *c, FusedBufs &B) {
         if (e == hipSuccess && ms < best_ms) { best_ms = ms; best = cand[k]; }
     }
     // FP64: the one-lane stage kernels with neighbour states preloaded, if the
-    // chosen set has any
+    // chosen set has any; then k_dv with the node tables in LDS, if it has that
     if (e == hipSuccess && c->precision == TDT_FP64 &&
         ((best & kSt1Mask) == kSt1Split || !(best & kStQuad))) {
         float ms = 0.f;
         e = time_set(c, B, best | kStPreload, kTuneIterations, nullptr, &ms);
         if (e == hipSuccess)
             e = hipMemcpyAsync(F.qn, B.qbak, qbytes, hipMemcpyDeviceToDevice, c->stream);
-        if (e == hipSuccess && ms < best_ms) best = best | kStPreload;
+        if (e == hipSuccess && ms < best_ms) {
+            best_ms = ms;
+            best = best | kStPreload;
+        }
+    }
+    if (e == hipSuccess && (best & kSt1Mask) == kSt1Split && node_lds_usable(c, B)) {
+        float ms = 0.f;
+        e = time_set(c, B, best | kDvNodeLds, kTuneIterations, nullptr, &ms);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(F.qn, B.qbak, qbytes, hipMemcpyDeviceToDevice, c->stream);
+        if (e == hipSuccess && ms < best_ms) best = best | kDvNodeLds;
     }
     c->profiling = prof;
     B.kset = best;

@@ -87,6 +87,15 @@ struct FDev {
     const int4 *vfl;
     float4 *vflux;
     const int4 *cvf;
+    // node tables of the wavefronts (FP64 single case, kDvNodeLds): wave wg (64
+    // cells of logical block wg / 4) lists the distinct nodes of its cells in
+    // bnode[wg * nl_cap ..], ascending and padded with the last to nl_cap (a
+    // multiple of 64, at most 256) entries; nloc[i] = cell i's four nodes' slots
+    // in its wave's list, 16 bits each: {slot0 | slot1 << 16, slot2 | slot3 << 16};
+    // nl_cap = 0: no tables
+    const int *bnode;
+    const int2 *nloc;
+    int nl_cap;
 };
 
 struct FusedBufs {
@@ -122,7 +131,12 @@ constexpr int kStQuad = 1 << 3;    // kq_stage<2..4> instead of k_rk<2..4>
 // per SIMD), one memory round trip instead of four: pays when the grid is too
 // small to give a SIMD more than one wavefront anyway.  FP32 always preloads.
 constexpr int kStPreload = 1 << 4;
-constexpr int kSetBits = kPreQuad | kSt1Mask | kStQuad | kStPreload;
+// FP64 k_dv (split stage 1, single case, LOCAL = 1): each wavefront copies the
+// node records of its cells into LDS with direct global-to-LDS loads, which hold
+// no registers in flight, and the viscous faces read them there instead of
+// gathering them face by face.
+constexpr int kDvNodeLds = 1 << 5;
+constexpr int kSetBits = kPreQuad | kSt1Mask | kStQuad | kStPreload | kDvNodeLds;
 // the uniform sets
 constexpr int kSetMerged = 0;
 constexpr int kSetSplit = kSt1Split;

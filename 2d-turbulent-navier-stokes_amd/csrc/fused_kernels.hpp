@@ -574,16 +574,106 @@ __global__ void __launch_bounds__(kBlock) k_pre(FDev F, Consts<double> K, Consts
 }
 
 // ---------------------------------------------------------------------------------
+// Node records of a wavefront's cells staged in LDS (FP64 k_dv with the node
+// tables): per wave, NR rounds of 64 slots of UN, VN | TN, RHON | X, Y as 16-byte
+// vectors and RN as two 32-bit halves, 56 bytes per slot, in dynamic LDS (the
+// block's four waves' regions one after the other).
+struct NodeLds {
+    double2 *uv, *tr, *xy;
+    unsigned *rlo, *rhi;
+};
+
+constexpr size_t kNodeLdsBytes = 3 * sizeof(double2) + 2 * sizeof(unsigned);   // per slot
+
+__device__ __forceinline__ NodeLds node_lds(int slots) {
+    extern __shared__ __align__(16) unsigned char node_smem[];
+    NodeLds s;
+    s.uv = reinterpret_cast<double2 *>(node_smem);
+    s.tr = s.uv + slots;
+    s.xy = s.tr + slots;
+    s.rlo = reinterpret_cast<unsigned *>(s.xy + slots);
+    s.rhi = s.rlo + slots;
+    return s;
+}
+
+// Direct global-to-LDS loads: lane l of the wave writes its 16 (4) bytes at
+// dst_wave + l, so dst_wave must be the same for the whole wave.
+template <class T>
+__device__ __forceinline__ void lds_copy16(const T *src, T *dst_wave) {
+    __builtin_amdgcn_global_load_lds(static_cast<const void *>(src),
+                                     (__attribute__((address_space(3))) void *)dst_wave, 16, 0, 0);
+}
+
+__device__ __forceinline__ void lds_copy4(const unsigned *src, unsigned *dst_wave) {
+    __builtin_amdgcn_global_load_lds(static_cast<const void *>(src),
+                                     (__attribute__((address_space(3))) void *)dst_wave, 4, 0, 0);
+}
+
+// The node ids of this wave's NR copy rounds: slot k = r * 64 + lane in round r
+// (F.nl_cap = NR * 64 slots per wave, the list padded to whole rounds).
+template <int NR>
+__device__ __forceinline__ void node_lds_ids(const FDev &F, int lb, int (&v)[NR]) {
+    const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+    const int *bn = F.bnode + ((size_t)lb * (kBlock / kWave) + w) * (NR * kWave) + lane;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) v[r] = bn[r * kWave];
+}
+
+// Waits for the ids (and whatever else is in flight: one wait for all, since a
+// wait between copies would also wait for the copies issued before it), then
+// issues this wave's copies; a round's 64 lanes write 64 consecutive slots.
+template <int NR>
+__device__ __forceinline__ void node_lds_copy(const FDev &F, const NodeLds &s, const int (&v)[NR]) {
+    const size_t m = F.m;
+    const double2 *uv = static_cast<const double2 *>(F.nrec);
+    const double2 *tr = uv + m;
+    const unsigned *rn =
+        reinterpret_cast<const unsigned *>(static_cast<const double *>(F.nrec) + 4 * m);
+    const double2 *xy = static_cast<const double2 *>(F.nxy);
+    const int w = threadIdx.x / kWave;
+    __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0); expcnt, lgkmcnt unconstrained
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        const int ws = (w * NR + r) * kWave;
+        lds_copy16(uv + v[r], s.uv + ws);
+        lds_copy16(tr + v[r], s.tr + ws);
+        lds_copy16(xy + v[r], s.xy + ws);
+        lds_copy4(rn + 2 * (size_t)v[r], s.rlo + ws);
+        lds_copy4(rn + 2 * (size_t)v[r] + 1, s.rhi + ws);
+    }
+}
+
+// This wave's copies are in LDS once its vector memory count is zero; a wave
+// reads only its own region, so no block barrier is needed.
+__device__ __forceinline__ void node_lds_wait() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 // k_dv: dissipation D and viscous fluxes VIS from Q^n in a kernel of their own
-// (the split variant; the merged variant forms them inside stage 1).
-template <class SD, class R, bool B6, bool BT>
-__global__ void __launch_bounds__(kBlock) k_dv(FDev F, Consts<double> K, Consts<R> KR) {
+// (the split variant; the merged variant forms them inside stage 1).  NR > 0
+// (k_dv_nodes: FP64 single case, LOCAL = 1): the viscous faces read the nodes
+// from the wave's LDS table, copied in NR rounds while the dissipation runs.
+template <class SD, class R, bool B6, bool BT, int NR>
+__device__ __forceinline__ void dv_cells(const FDev &F, Consts<double> K, Consts<R> KR) {
+    constexpr bool NL = NR > 0;
+    static_assert(!NL || (!BT && sizeof(SD) == 8 && NR <= 4), "node tables: FP64 single case");
     int i;
+    bool active = true;
+    NodeLds nls{};
+    int nid[NL ? NR : 1];
     if constexpr (BT) {
         const int4 bl = F.cblk[cell_block(F)];
         if ((int)threadIdx.x >= bl.y) return;
         case_consts<BT>(F, bl.z, K, KR);
         i = bl.x + threadIdx.x;
+    } else if constexpr (NL) {
+        const int lb = cell_block(F);
+        if (lb * kBlock >= F.n) return;   // a padding block (uniform)
+        nls = node_lds(NR * kBlock);
+        node_lds_ids<NR>(F, lb, nid);
+        // lanes past the last cell compute cell n-1 and store nothing, so that
+        // every lane of a wave takes part in its copies
+        i = lb * kBlock + threadIdx.x;
+        active = i < F.n;
+        if (!active) i = F.n - 1;
     } else {
         i = cell_block(F) * kBlock + threadIdx.x;
         if (i >= F.n) return;
@@ -591,25 +681,40 @@ __global__ void __launch_bounds__(kBlock) k_dv(FDev F, Consts<double> K, Consts<
     const int nt = F.nt, n = F.n;
     const double *Q = F.qn;
     // DT enters D only through VOL/DT: with LOCAL = 1 the record holds it, with
-    // LOCAL = 0 the record holds VOL and DT is the global minimum
-    const double dti = K.local ? 0.0 : __longlong_as_double((long long)*F.dtslot);
-    if (!K.local) F.dt[i] = dti;
-    int4 nbt = F.nct[i], nd4 = F.nod4[i];
+    // LOCAL = 0 the record holds VOL and DT is the global minimum.  NL runs with
+    // LOCAL = 1 only: a DTMIN load under a branch would make the join wait for
+    // every load in flight, the copies included.
+    const int local = NL ? 1 : K.local;
+    const double dti = local ? 0.0 : __longlong_as_double((long long)*F.dtslot);
+    if (!local) F.dt[i] = dti;
+    int4 nbt = F.nct[i], nd4;
+    if constexpr (NL) {
+        // slots in the block's LDS: the wave's region, then the slot in its list
+        const int2 sl = F.nloc[i];
+        const int wb = (threadIdx.x / kWave) * (NR * kWave);
+        nd4 = make_int4(wb + (sl.x & 0xFFFF), wb + ((unsigned)sl.x >> 16), wb + (sl.y & 0xFFFF),
+                        wb + ((unsigned)sl.y >> 16));
+    } else {
+        nd4 = F.nod4[i];
+    }
     // raw[f]: neighbour id, with the BC type in the top bits for a ghost
     const int raw[4] = {nbt.x, nbt.y, nbt.z, nbt.w};
     const int nd[4] = {nd4.x, nd4.y, nd4.z, nd4.w};
     // ---- dissipation (solver_state.f90:183-223); ghost neighbour = parent copy
     double ci[5];
     for (int q = 0; q < 5; ++q) ci[q] = Q[q * nt + i];
+    if constexpr (NL) node_lds_copy<NR>(F, nls, nid);   // after the cell's own loads
     R dsum[5];
     {
         const int nbr[4] = {raw[0] & kIdMask, raw[1] & kIdMask, raw[2] & kIdMask, raw[3] & kIdMask};
-        dissipation_rec<R, SD>(F, i, nbr, ci, K.local, dti, dsum);
+        dissipation_rec<R, SD>(F, i, nbr, ci, local, dti, dsum);
         // stored now in either form (FP32-mixed overwrites it with D + VIS
         // below): the store keeps the viscous part's loads after this point,
         // which holds the register count down
-        for (int q = 0; q < 5; ++q) stv<SD>(F.dd, (size_t)q * n + i, dsum[q]);
+        if (active)
+            for (int q = 0; q < 5; ++q) stv<SD>(F.dd, (size_t)q * n + i, dsum[q]);
     }
+    if constexpr (NL) node_lds_wait();
     // ---- viscous fluxes (viscous.f90:62-70, 130-217)
     R ua, va, ta, pa;
     R rhoa = (R)ci[0];
@@ -648,15 +753,39 @@ __global__ void __launch_bounds__(kBlock) k_dv(FDev F, Consts<double> K, Consts<
         }
         R rc = fdiv(c5c, rhoc);
         R sx, sy;
-        face_sc<R, SD, true>(F, f, i, bnode, dnode, sx, sy);
-        ViscFace<R> o = viscous_face_rec<R, SD>(F, sx, sy, xa, ya, ua, va, ta, rhoa, ra, bnode,
-                                                dnode, (R)ld<SD>(F.gxcc, c),
-                                                (R)ld<SD>(F.gycc, c), uc, vc, tc, rhoc, rc, KR);
+        ViscFace<R> o;
+        if constexpr (NL) {
+            // the node records from LDS; the face vector as face_sc forms it
+            const double2 bxy = nls.xy[bnode], dxy = nls.xy[dnode];
+            const double2 buv = nls.uv[bnode], duv = nls.uv[dnode];
+            const double2 btr = nls.tr[bnode], dtr = nls.tr[dnode];
+            const double brn = __hiloint2double((int)nls.rhi[bnode], (int)nls.rlo[bnode]);
+            const double drn = __hiloint2double((int)nls.rhi[dnode], (int)nls.rlo[dnode]);
+            if (F.sc_nodes == 2) {
+                sx = (dxy.y - bxy.y) * (bxy.y + dxy.y) * 0.5;
+                sy = -((dxy.x - bxy.x) * (bxy.y + dxy.y) * 0.5);
+            } else if (F.sc_nodes == 1) {
+                sx = dxy.y - bxy.y;
+                sy = -(dxy.x - bxy.x);
+            } else {
+                face_sc<R, SD, false>(F, f, i, 0, 0, sx, sy);
+            }
+            o = viscous_face<R>(sx, sy, xa, ya, bxy.x, bxy.y, (R)ld<SD>(F.gxcc, c),
+                                (R)ld<SD>(F.gycc, c), dxy.x, dxy.y, ua, va, ta, rhoa, ra, buv.x,
+                                buv.y, btr.x, btr.y, brn, uc, vc, tc, rhoc, rc, duv.x, duv.y,
+                                dtr.x, dtr.y, drn, KR);
+        } else {
+            face_sc<R, SD, true>(F, f, i, bnode, dnode, sx, sy);
+            o = viscous_face_rec<R, SD>(F, sx, sy, xa, ya, ua, va, ta, rhoa, ra, bnode, dnode,
+                                        (R)ld<SD>(F.gxcc, c), (R)ld<SD>(F.gycc, c), uc, vc, tc,
+                                        rhoc, rc, KR);
+        }
         vu = vu + o.u0 + o.u1;
         vv = vv + o.v0 + o.v1;
         vt = vt + o.t0 + o.t1;
         vr = vr + o.r0 + o.r1;
     }
+    if (!active) return;
     if constexpr (kCombinedDV<SD>) {
         const R e[5] = {dsum[0], dsum[1] + vu, dsum[2] + vv, dsum[3] + vt, dsum[4] + vr};
         for (int q = 0; q < 5; ++q) stv<SD>(F.dd, (size_t)q * n + i, e[q]);
@@ -666,6 +795,21 @@ __global__ void __launch_bounds__(kBlock) k_dv(FDev F, Consts<double> K, Consts<
         stv<SD>(F.vis, (size_t)2 * n + i, vt);
         stv<SD>(F.vis, (size_t)3 * n + i, vr);
     }
+}
+
+
+template <class SD, class R, bool B6, bool BT>
+__global__ void __launch_bounds__(kBlock) k_dv(FDev F, Consts<double> K, Consts<R> KR) {
+    dv_cells<SD, R, B6, BT, 0>(F, K, KR);
+}
+
+// k_dv with the wave node tables (NR copy rounds): bounded to three waves per
+// SIMD, the register count of k_dv itself (the compiler then spills 20 bytes per
+// lane; unbounded it takes 171 registers and two waves per SIMD).
+template <bool B6, int NR>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3)))
+k_dv_nodes(FDev F, Consts<double> K, Consts<double> KR) {
+    dv_cells<double, double, B6, false, NR>(F, K, KR);
 }
 
 // ---------------------------------------------------------------------------------
@@ -991,12 +1135,37 @@ namespace {
 // Launches one kernel; when profiling, with start/stop events that the runtime
 // stamps at the kernel's own begin and end (same durations a kernel trace shows).
 template <class K, class... Args>
-void launch(tdt_ctx *c, FusedBufs &B, int slot, K kern, unsigned grid, Args... args) {
+void launch_shm(tdt_ctx *c, FusedBufs &B, int slot, K kern, unsigned grid, size_t shm,
+                Args... args) {
     if (c->profiling && B.have_events)
-        hipExtLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, c->stream, B.ev[2 * slot],
+        hipExtLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), shm, c->stream, B.ev[2 * slot],
                               B.ev[2 * slot + 1], 0, args...);
     else
-        hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, c->stream, args...);
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), shm, c->stream, args...);
+}
+
+template <class K, class... Args>
+void launch(tdt_ctx *c, FusedBufs &B, int slot, K kern, unsigned grid, Args... args) {
+    launch_shm(c, B, slot, kern, grid, 0, args...);
+}
+
+// k_dv with the wave node tables can run: FP64 single case (the tables exist)
+// with LOCAL = 1.
+inline bool node_lds_usable(const tdt_ctx *c, const FusedBufs &B) {
+    return B.F.nl_cap > 0 && B.F.ncases == 0 && c->K.local;
+}
+
+// k_dv_nodes for the table's rounds per wave (nl_cap / 64, 1..4).
+template <bool B6>
+void launch_dv_nodes(tdt_ctx *c, FusedBufs &B) {
+    const FDev &F = B.F;
+    const size_t shm = (size_t)F.nl_cap * (kBlock / kWave) * kNodeLdsBytes;
+    switch (F.nl_cap / kWave) {
+    case 1: launch_shm(c, B, 1, k_dv_nodes<B6, 1>, F.ncb8, shm, F, c->K, c->K); break;
+    case 2: launch_shm(c, B, 1, k_dv_nodes<B6, 2>, F.ncb8, shm, F, c->K, c->K); break;
+    case 3: launch_shm(c, B, 1, k_dv_nodes<B6, 3>, F.ncb8, shm, F, c->K, c->K); break;
+    default: launch_shm(c, B, 1, k_dv_nodes<B6, 4>, F.ncb8, shm, F, c->K, c->K);
+    }
 }
 
 // One-lane stage kernel, with the neighbour states preloaded or not (FP64; FP32
@@ -1045,6 +1214,13 @@ hipError_t enqueue_iteration(tdt_ctx *c, FusedBufs &B, const Consts<R> &KR) {
     }
     switch (ks & kSt1Mask) {
     case kSt1Split:
+        if constexpr (sizeof(SD) == 8 && !BT) {
+            if ((ks & kDvNodeLds) && node_lds_usable(c, B)) {
+                launch_dv_nodes<B6>(c, B);
+                launch_rk<R, double, SS, SD, 1, false, B6, BT>(c, B, 2, pl, KR, 0.25);
+                break;
+            }
+        }
         launch(c, B, 1, k_dv<SD, R, B6, BT>, nbc, F, c->K, KR);
         launch_rk<R, double, SS, SD, 1, false, B6, BT>(c, B, 2, pl, KR, 0.25);
         break;

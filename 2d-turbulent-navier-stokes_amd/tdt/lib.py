@@ -393,14 +393,15 @@ class Solver:
         """The fused kernel set in use: "merged", "split", "quad", "face" (FP32-mixed:
         each viscous face evaluated once), or for a mixed
         choice e.g. "pre=quad,stage1=merged,stages=lane", with "+preload" when the
-        FP64 one-lane stage kernels load all neighbour states up front ("" before
-        the first tdt_iterate)."""
+        FP64 one-lane stage kernels load all neighbour states up front and "+nodes"
+        when the FP64 dissipation / viscous kernel copies its node records into LDS
+        ("" before the first tdt_iterate)."""
         out = np.zeros(1, dtype=np.int32)
         self._check(self.L.tdt_kernel_set(self.h, _i(out)))
         v = int(out[0])
         if v < 0:
             return ""
-        pl = "+preload" if v & 16 else ""     # FP64 one-lane stages preload neighbours
+        pl = ("+preload" if v & 16 else "") + ("+nodes" if v & 32 else "")
         v &= 15
         if v in self._UNIFORM:
             return self._UNIFORM[v] + pl

@@ -58,6 +58,9 @@ def alg_bytes_per_cell(precision: str, m_over_n: float, local: int, kset: str = 
     FP64 k_dv forms the face vectors from the node coordinates it gathers for
     the viscous faces (SC is not streamed) and, with LOCAL = 1, reads no DT
     (the cell records carry VOL/DT).
+    With the wave node tables ("+nodes"), k_dv reads each cell's four 16-bit
+    slots (8 bytes) instead of its node ids (16) and its wave's list of node ids,
+    3 rounds of 64 ids per 64 cells on the structured grids (12 bytes per cell).
     The merged and quad kernel sets have no k_dv launch (stage 1 forms the
     dissipation and viscous fluxes itself); the face set's k_vface (slot k_dv)
     evaluates each of the f_over_n faces per cell once: a 16-byte face record
@@ -74,6 +77,8 @@ def alg_bytes_per_cell(precision: str, m_over_n: float, local: int, kset: str = 
     base = dt + nct + geo + frozen             # every stage: DT, ids, geometry, D (+VIS)
     dv = ((0 if local else dt) + nct + nod + q + rec + (2 if precision == "fp64" else 10) * g
           + nodes + frozen)
+    if "+nodes" in kset:
+        dv += 8 + 12 - nod
     st1 = q + base + 5 * s
     st23 = 5 * s + q + base + 5 * s
     st4 = 5 * s + q + base + q

@@ -272,6 +272,11 @@ int tdt_kernel_times(tdt_ctx *ctx, double *times_ms);
                                          stages 2-4) request all neighbour states up
                                          front: one memory round trip instead of four,
                                          at one wavefront per SIMD                     */
+#define TDT_KERNELS_NODE_LDS 32       /* FP64 split stage 1, single case, LOCAL = 1:
+                                         each wavefront of the dissipation / viscous
+                                         kernel copies its cells' node records into
+                                         LDS (direct global-to-LDS loads) instead of
+                                         gathering them face by face                   */
 #define TDT_KERNELS_MERGED 0          /* uniform sets: one lane per cell, D/VIS in stage 1 */
 #define TDT_KERNELS_SPLIT 2
 #define TDT_KERNELS_QUAD 13

@@ -182,8 +182,9 @@ def test_kernel_sets_identical(golden, case, precision, monkeypatch):
     g = golden(case)
     runs = {}
     # the uniform sets and mixed choices (pre-pass quad; stage 1 quad; stages 2-4 quad;
-    # merged and split with the FP64 one-lane stages preloading their neighbours)
-    extra = ("7", "14") if precision == "fp32-mixed" else ()   # face with quad pre / stages
+    # merged and split with the FP64 one-lane stages preloading their neighbours;
+    # FP64 split with the wave node tables in LDS, alone and with preloading)
+    extra = ("7", "14") if precision == "fp32-mixed" else ("34", "50")
     for ks in tdt.Solver.KERNEL_SETS + ("1", "4", "8", "5", "16", "18") + extra:
         if ks == "face" and precision == "fp64":
             continue          # FP64 keeps the reference's per-cell viscous evaluation
@@ -193,6 +194,8 @@ def test_kernel_sets_identical(golden, case, precision, monkeypatch):
             l2, linf, _ = s.iterate(40)
             if not ks.isdigit():
                 assert s.kernel_set() == ks
+            if ks in ("34", "50"):   # the node tables need LOCAL = 1 (else plain split)
+                assert s.kernel_set().endswith("+nodes") == bool(g.flow.local), s.kernel_set()
             runs[s.kernel_set()] = (s.get_state(), l2, linf)
     base = runs["merged"]
     for ks, (st, l2, linf) in runs.items():
@@ -259,13 +262,15 @@ def _routine_iterations(s, k):
         s.monitor()
 
 
-@pytest.mark.parametrize("case", ["t2d_ramp", "case1_axi"])
-def test_split_set_surface_vectors_follow_set_array(golden, case, monkeypatch):
+@pytest.mark.parametrize("kset", ["split", "34"])
+@pytest.mark.parametrize("case", ["t2d_ramp", "case1_axi", "taxi_duct"])
+def test_split_set_surface_vectors_follow_set_array(golden, case, kset, monkeypatch):
     """FP64 k_dv forms each face's surface vector from its nodes while SC is the
     nodes' (planar and axisymmetric forms); once tdt_set_array writes SC it must
-    read SC again, like the routine path: both stay bitwise equal."""
+    read SC again, like the routine path: both stay bitwise equal (34: with the
+    wave node tables in LDS where LOCAL = 1)."""
     import tdt
-    monkeypatch.setenv("TDT_KERNELS", "split")
+    monkeypatch.setenv("TDT_KERNELS", kset)
     g = golden(case)
     with tdt.Solver(g.mesh, g.flow, "fp64") as a, tdt.Solver(g.mesh, g.flow, "fp64") as b:
         a.cold_start()
@@ -352,13 +357,15 @@ def full_ramp():
     return mk_mesh(), mk_flow()
 
 
-def test_full_size_ramp_fused_equals_routine_path(full_ramp, monkeypatch):
+@pytest.mark.parametrize("kset", ["split", "34"])
+def test_full_size_ramp_fused_equals_routine_path(full_ramp, kset, monkeypatch):
     """At the full 1.05 M-cell size (boundary-last cell order), two fused
     iterations equal two iterations of the routine entry points bit for bit,
-    norms included (the reference's own fixtures stop at 50 k cells)."""
+    norms included (the reference's own fixtures stop at 50 k cells); 34: the
+    split set with the wave node tables in LDS."""
     import tdt
     mesh, flow = full_ramp
-    monkeypatch.setenv("TDT_KERNELS", "split")
+    monkeypatch.setenv("TDT_KERNELS", kset)
     with tdt.Solver(mesh, flow, "fp64") as a, tdt.Solver(mesh, flow, "fp64") as b:
         assert a.ordering_used == 3        # boundary-last at this size
         a.cold_start()
