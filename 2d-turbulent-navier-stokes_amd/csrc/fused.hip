@@ -270,39 +270,10 @@ int fused_setup(tdt_ctx *c) {
         F.cvf = d_cvf;
     }
     if (c->precision == TDT_FP64 && !c->batch && ok) {
-        // node tables of the wavefronts (kDvNodeLds): the distinct nodes of each
-        // wave's 64 cells, ascending, and each cell's four slots in its wave's list
-        const int nwv = B->nb_cells * (kBlock / kWave);
-        auto wave_nodes = [&](int w, std::vector<int> &v) {
-            v.clear();
-            for (int i = w * kWave; i < std::min(D.n, (w + 1) * kWave); ++i)
-                for (int q = 0; q < 4; ++q) v.push_back(pm.nod4[4 * i + q]);
-            std::sort(v.begin(), v.end());
-            v.erase(std::unique(v.begin(), v.end()), v.end());
-            if (v.empty()) v.push_back(0);   // a wave past the last cell: any node
-        };
-        std::vector<int> v;
-        size_t cap = 0;
-        for (int w = 0; w < nwv; ++w) {
-            wave_nodes(w, v);
-            cap = std::max(cap, v.size());
-        }
-        cap = (cap + kWave - 1) / kWave * kWave;   // whole 64-lane copy rounds, <= 256
-        std::vector<int> bnode((size_t)nwv * cap), nloc((size_t)2 * D.n);
-        for (int w = 0; w < nwv; ++w) {
-            wave_nodes(w, v);
-            std::copy(v.begin(), v.end(), bnode.begin() + (size_t)w * cap);
-            std::fill(bnode.begin() + (size_t)w * cap + v.size(),
-                      bnode.begin() + (size_t)(w + 1) * cap, v.back());
-            for (int i = w * kWave; i < std::min(D.n, (w + 1) * kWave); ++i) {
-                int sl[4];
-                for (int q = 0; q < 4; ++q)
-                    sl[q] = (int)(std::lower_bound(v.begin(), v.end(), pm.nod4[4 * i + q]) -
-                                  v.begin());
-                nloc[2 * i] = sl[0] | sl[1] << 16;
-                nloc[2 * i + 1] = sl[2] | sl[3] << 16;
-            }
-        }
+        // node tables of the wavefronts (kDvNodeLds)
+        std::vector<int> bnode, nloc;
+        const int cap = wave_node_tables(pm.nod4, D.n, B->nb_cells * (kBlock / kWave), bnode,
+                                         nloc);
         int *d_bnode = dalloc<int>(*B, bnode.size());
         int *d_nloc = dalloc<int>(*B, nloc.size());
         ok = d_bnode && d_nloc &&
@@ -312,7 +283,7 @@ int fused_setup(tdt_ctx *c) {
                  hipSuccess;
         F.bnode = d_bnode;
         F.nloc = reinterpret_cast<const int2 *>(d_nloc);
-        F.nl_cap = (int)cap;
+        F.nl_cap = cap;
     }
     int *d_nct = dalloc<int>(*B, nct.size());
     int *d_ncon = dalloc<int>(*B, ncon.size());

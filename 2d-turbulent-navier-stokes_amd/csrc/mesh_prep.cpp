@@ -348,4 +348,41 @@ std::string prepare_mesh(const tdt_mesh &ms, int ordering, PreparedMesh &pm, int
     return "";
 }
 
+int wave_node_tables(const std::vector<int> &nod4, int n, int nwaves, std::vector<int> &bnode,
+                     std::vector<int> &nloc) {
+    constexpr int kWaveCells = 64;
+    auto wave_nodes = [&](int w, std::vector<int> &v) {
+        v.clear();
+        for (int i = w * kWaveCells; i < std::min(n, (w + 1) * kWaveCells); ++i)
+            for (int q = 0; q < 4; ++q) v.push_back(nod4[(size_t)4 * i + q]);
+        std::sort(v.begin(), v.end());
+        v.erase(std::unique(v.begin(), v.end()), v.end());
+        if (v.empty()) v.push_back(0);   // a wave past the last cell: any node
+    };
+    std::vector<int> v;
+    size_t cap = 0;
+    for (int w = 0; w < nwaves; ++w) {
+        wave_nodes(w, v);
+        cap = std::max(cap, v.size());
+    }
+    cap = (cap + kWaveCells - 1) / kWaveCells * kWaveCells;   // whole 64-lane copy rounds
+    bnode.assign((size_t)nwaves * cap, 0);
+    nloc.assign((size_t)2 * n, 0);
+    for (int w = 0; w < nwaves; ++w) {
+        wave_nodes(w, v);
+        std::copy(v.begin(), v.end(), bnode.begin() + (size_t)w * cap);
+        std::fill(bnode.begin() + (size_t)w * cap + v.size(), bnode.begin() + (size_t)(w + 1) * cap,
+                  v.back());
+        for (int i = w * kWaveCells; i < std::min(n, (w + 1) * kWaveCells); ++i) {
+            int sl[4];
+            for (int q = 0; q < 4; ++q)
+                sl[q] = (int)(std::lower_bound(v.begin(), v.end(), nod4[(size_t)4 * i + q]) -
+                              v.begin());
+            nloc[(size_t)2 * i] = sl[0] | sl[1] << 16;
+            nloc[(size_t)2 * i + 1] = sl[2] | sl[3] << 16;
+        }
+    }
+    return (int)cap;
+}
+
 }  // namespace tdt

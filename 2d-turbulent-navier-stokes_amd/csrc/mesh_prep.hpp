@@ -54,4 +54,14 @@ std::string prepare_mesh(const tdt_mesh &mesh, int ordering, PreparedMesh &out, 
 // Bandwidth of a cell order: max |pos(i) - pos(j)| over interior neighbour pairs.
 int order_bandwidth(int n, const std::vector<int> &nc_old, const std::vector<int> &old2new);
 
+// Node lists of the wavefronts (each 64 consecutive cells of the prepared order)
+// for kernels that copy their cells' node records into LDS: bnode[w * cap + k],
+// k < cap, = the distinct nodes of wave w's cells in ascending order, padded
+// with the last (a wave past the last cell lists node 0); nloc[2i], nloc[2i+1] =
+// cell i's four nodes' slots in its wave's list, 16 bits each ({slot0 | slot1 <<
+// 16, slot2 | slot3 << 16}).  nwaves >= the waves the n cells fill.  Returns cap,
+// the longest list rounded up to a multiple of 64 (at most 256).
+int wave_node_tables(const std::vector<int> &nod4, int n, int nwaves, std::vector<int> &bnode,
+                     std::vector<int> &nloc);
+
 }  // namespace tdt

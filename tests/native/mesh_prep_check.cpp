@@ -5,7 +5,9 @@
 // neles, nghosts, double x_mm[nods], y_mm[nods], int32 nod[4][neles],
 // nc[4][neles], parent[nghosts], ghost[nghosts], type[nghosts], side[nghosts].
 // Output, one line per mesh and ordering: "mesh <k> order <o> code <c> n <n> g
-// <g> m <m> bw <bandwidth> used <ordering> sum <checksum of the tables>".
+// <g> m <m> bw <bandwidth> used <ordering> sum <checksum of the tables> nlcap
+// <wave node-list length> nlok <1 if the wave node tables hold every cell's
+// nodes at its slots, in sorted, padded lists>".
 #include <cstdint>
 #include <cstdio>
 #include <vector>
@@ -30,6 +32,33 @@ unsigned long long checksum(const tdt::PreparedMesh &pm) {
     add(pm.gparent); add(pm.gside); add(pm.gtype); add(pm.gopp);
     add(pm.nptr); add(pm.nlist); add(pm.novr); add(pm.g6ptr); add(pm.g6cell);
     return s;
+}
+
+// The wave node tables (wave_node_tables) checked against the cells' nodes.
+bool wave_tables_ok(const tdt::PreparedMesh &pm, int *cap_out) {
+    const int nwaves = (pm.n + 255) / 256 * 4;   // whole blocks of four waves
+    std::vector<int> bnode, nloc;
+    const int cap = tdt::wave_node_tables(pm.nod4, pm.n, nwaves, bnode, nloc);
+    *cap_out = cap;
+    if (cap <= 0 || cap % 64 != 0 || cap > 256) return false;
+    if (bnode.size() != (size_t)nwaves * cap || nloc.size() != (size_t)2 * pm.n) return false;
+    for (int w = 0; w < nwaves; ++w) {
+        const int *row = &bnode[(size_t)w * cap];
+        int len = 1;
+        while (len < cap && row[len] > row[len - 1]) ++len;   // the ascending part
+        for (int k = len; k < cap; ++k)
+            if (row[k] != row[len - 1]) return false;         // padded with the last
+        for (int k = 0; k < len; ++k)
+            if (row[k] < 0 || row[k] >= pm.m) return false;
+    }
+    for (int i = 0; i < pm.n; ++i) {
+        const int w = i / 64;
+        const int sl[4] = {nloc[2 * i] & 0xFFFF, (int)((unsigned)nloc[2 * i] >> 16),
+                           nloc[2 * i + 1] & 0xFFFF, (int)((unsigned)nloc[2 * i + 1] >> 16)};
+        for (int q = 0; q < 4; ++q)
+            if (sl[q] >= cap || bnode[(size_t)w * cap + sl[q]] != pm.nod4[4 * i + q]) return false;
+    }
+    return true;
 }
 
 }  // namespace
@@ -62,9 +91,11 @@ int main(int argc, char **argv) {
             tdt::PreparedMesh pm;
             int code = 0;
             tdt::prepare_mesh(m, o, pm, &code);
-            printf("mesh %d order %d code %d n %d g %d m %d bw %d used %d sum %llu\n", k, o, code,
-                   pm.n, pm.g, pm.m, pm.bandwidth, pm.ordering_used,
-                   code == 0 ? checksum(pm) : 0ULL);
+            int cap = 0;
+            const bool nlok = code == 0 && wave_tables_ok(pm, &cap);
+            printf("mesh %d order %d code %d n %d g %d m %d bw %d used %d sum %llu nlcap %d "
+                   "nlok %d\n", k, o, code, pm.n, pm.g, pm.m, pm.bandwidth, pm.ordering_used,
+                   code == 0 ? checksum(pm) : 0ULL, cap, nlok ? 1 : 0);
         }
     }
     fclose(f);

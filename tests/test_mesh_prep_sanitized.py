@@ -1,5 +1,6 @@
 """The host-side mesh preparation (csrc/mesh_prep.cpp: validation, renumbering,
-ghost tables, node-gather lists, BC type 6 port columns) under AddressSanitizer
+ghost tables, node-gather lists, BC type 6 port columns, the wave node tables)
+under AddressSanitizer
 and UndefinedBehaviorSanitizer, on the CPU: a g++ build of it with a small
 driver (tests/native/mesh_prep_check.cpp) prepares fixtures and generated grids
 in every ordering, and malformed grids, which it must reject with an error code
@@ -67,6 +68,9 @@ def test_valid_grids_prepare_cleanly(checker, tmp_path, golden):
         # RCM is kept only where it narrows the bandwidth of grid.in's numbering
         by = {r["order"]: r for r in mine}
         assert by[1]["bw"] <= by[0]["bw"] or by[1]["used"] == 0
+        # the wave node tables (LDS node copies of FP64 k_dv): every cell's four nodes at
+        # its slots of its wave's sorted, padded list, at most four 64-lane rounds
+        assert all(r["nlok"] == 1 and 64 <= r["nlcap"] <= 256 for r in mine), mine
 
 
 def _bad(m, what):
