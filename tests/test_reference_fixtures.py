@@ -33,3 +33,33 @@ def test_reference_ramp_type3_goes_nonfinite():
     assert linf[3699, 3] > 20.0 * linf[3199, 3]
     # the early history settles (the instability is not a start-up transient)
     assert linf[2999, 3] < 0.2 * linf[0, 3]
+
+
+DUCT_CONVERGED = os.path.join(GOLDEN, "duct_193x65_converged.npz")
+
+
+@pytest.mark.skipif(not os.path.exists(DUCT_CONVERGED),
+                    reason="duct_193x65_converged fixture not generated")
+def test_reference_adjoint_duct_converges_at_cfl_half():
+    """The 193x65 adjoint duct at CFL 0.5 reaches a steady state in the
+    reference's own arithmetic: its L2 history drops below 1e-8 of the first
+    iterations' near 700,000 iterations and below 1e-10 by 850,000, so the
+    central differences of its J are those of the steady state.  The moved runs'
+    J straddle the base run's (second-order differences small against the
+    first-order ones: the differences resolve the derivative)."""
+    with np.load(DUCT_CONVERGED) as z:
+        every = z["hist_l2_every1000"]
+        head = z["hist_l2_head"]
+        nrun = int(z["nrun"])
+        jb = float(z["J_base"])
+        jp = {k: (float(z[f"J_{k}+"]), float(z[f"J_{k}-"])) for k in ("p0", "t0", "bump3")}
+    assert nrun == 850000 and len(every) == nrun // 1000
+    drop = every / head[:10].max(axis=0)
+    print(f"reference L2 drop at 400k {drop[399].max():.2e}, 700k {drop[699].max():.2e}, "
+          f"850k {drop[-1].max():.2e}")
+    assert drop[709:].max() < 1e-8 and drop[-1].max() < 1e-10
+    assert drop[399].max() > 1e-7            # not settled before 400,000 iterations
+    for key, (a, b) in jp.items():
+        first = abs(a - b) / 2
+        second = abs((a + b) / 2 - jb)
+        assert first > 0 and second < 1e-3 * first, (key, first, second)
