@@ -432,3 +432,53 @@ def test_new_rinit_after_iterate_is_used(golden):
         a.set_state(c, rinit * 1e3)
         a.iterate(3, history=False)
         assert not np.array_equal(a.get_state(), b.get_state())
+
+
+def _permuted(mesh, seed):
+    """The same grid with its cells in a random order (grid.in order), so that a
+    wavefront's 64 consecutive cells share few nodes."""
+    from tdt import caseio
+    n = mesh.neles
+    p = np.random.default_rng(seed).permutation(n)          # new -> old
+    q = np.empty(n, np.int64)
+    q[p] = np.arange(n)                                     # old -> new
+    nc = mesh.nc[:, p].copy()
+    cell = (nc >= 1) & (nc <= n)
+    nc[cell] = q[nc[cell] - 1] + 1
+    return caseio.Mesh(x_mm=mesh.x_mm, y_mm=mesh.y_mm, nod=mesh.nod[:, p].copy(), nc=nc,
+                       parent=(q[mesh.parent - 1] + 1).astype(mesh.parent.dtype),
+                       ghost=mesh.ghost, btype=mesh.btype, side=mesh.side)
+
+
+def _copy_rounds(mesh):
+    """64-lane copy rounds per wavefront of k_dv_nodes for grid.in's cell order."""
+    nod = mesh.nod.T
+    longest = max(len(np.unique(nod[w:w + 64])) for w in range(0, mesh.neles, 64))
+    return -(-longest // 64)
+
+
+@pytest.mark.parametrize("grid,rounds", [("channel_6x2", 1), ("channel_16x8", 2),
+                                         ("channel_12x70", 3), ("channel_40x20_shuffled", 4)])
+def test_dv_node_tables_every_round_count(grid, rounds, monkeypatch):
+    """k_dv_nodes is compiled for 1-4 copy rounds per wavefront (64-256 node
+    slots); each equals the split set bit for bit, norms included (cell order
+    kept as given, so a shuffled grid fills four rounds)."""
+    import tdt
+    from tdt import gridgen
+    flow = gridgen.bl_inlet_params(gridgen.flow_channel())
+    nx, ny = (int(v) for v in grid.split("_")[1].split("x"))
+    mesh = gridgen.channel(nx=nx, ny=ny)
+    if grid.endswith("shuffled"):
+        mesh = _permuted(mesh, 7)
+    assert _copy_rounds(mesh) == rounds
+    runs = {}
+    for ks in ("2", "34"):
+        monkeypatch.setenv("TDT_KERNELS", ks)
+        with tdt.Solver(mesh, flow, "fp64", "input") as s:
+            s.cold_start()
+            l2, linf, bad = s.iterate(30)
+            runs[ks] = (s.get_state(), l2, linf, s.kernel_set())
+    assert runs["34"][3] == "split+nodes", runs["34"][3]
+    a, b = runs["2"], runs["34"]
+    assert np.array_equal(a[0], b[0]), f"max diff {np.max(np.abs(a[0] - b[0]))}"
+    assert np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
