@@ -100,7 +100,7 @@ def alg_bytes_per_cell(precision: str, m_over_n: float, local: int, kset: str = 
 # every per-cell array counted once per pass (time step, dissipation, node
 # averages, four stages) that reads or writes it, at its storage width; FP64
 # 860 + 81 * 8, FP32-mixed 860 + 81 * 4.
-SURVEY_BYTES = {"fp64": 1508.0, "fp32-mixed": 1184.0}
+PER_PASS_BYTES = {"fp64": 1508.0, "fp32-mixed": 1184.0}
 
 
 def load_traffic(workload: str, precision: str, kset: str):
@@ -137,9 +137,9 @@ def roofline(workload, precision, mesh, flow, kt, kset, elapsed, steps):
             "alg_bytes_per_cell_iter": round(sum(b.values()), 1),
             "iteration_alg_GBs": round(sum(b.values()) * rate / 1e9, 1),
             "iteration_frac": round(sum(b.values()) * rate / 1e9 / HBM_PEAK_GBS, 4),
-            "survey_bytes_per_cell_iter": SURVEY_BYTES[precision],
-            "iteration_alg_GBs_survey": round(SURVEY_BYTES[precision] * rate / 1e9, 1),
-            "iteration_frac_survey": round(SURVEY_BYTES[precision] * rate / 1e9 / HBM_PEAK_GBS,
+            "per_pass_bytes_per_cell_iter": PER_PASS_BYTES[precision],
+            "iteration_alg_GBs_per_pass": round(PER_PASS_BYTES[precision] * rate / 1e9, 1),
+            "iteration_frac_per_pass": round(PER_PASS_BYTES[precision] * rate / 1e9 / HBM_PEAK_GBS,
                                            4)}
 
 
