@@ -7,6 +7,8 @@ Fixtures (tests/golden/make_golden.py, written by the reference solver):
   duct_385x129_2000   the 385x129 axisymmetric duct, 2000 iterations, every cell
   duct_385x129_20000  the same duct over its NITER = 20000: L2 history, every cell,
                       and the reference executable's periodic restart file
+  ramp_2049x513_t7_20000  the 2049x513 ramp with its type-7 inflow over 20000
+                      iterations: L2 history, every 97th cell
 Bar for FP64: per-iteration L2 (and Linf) history and final rho, u, v, p, nu~
 within 1e-10 relative of the reference.  FP32-mixed is held to 1e-5 in pressure
 against the reference's state at the run length.
@@ -209,3 +211,48 @@ def test_ramp_type3_blowup_location(ramp_t3):
         assert len(cells) > 0, prec
         assert i.min() == 0 and i.max() <= 4 and j.min() >= 480, (prec, i.max(), j.min())
 
+
+
+RAMP_T7 = "ramp_2049x513_t7_20000"
+needs_ramp_t7 = pytest.mark.skipif(not os.path.exists(os.path.join(GOLDEN, RAMP_T7 + ".npz")),
+                                   reason=RAMP_T7 + " fixture not generated")
+
+
+@pytest.fixture(scope="module")
+def ramp_t7_runs(golden):
+    """The 2049x513 ramp with its type-7 inflow over 20000 iterations from the
+    cold start, in both precisions (the reference's run: tests/golden)."""
+    g = golden(RAMP_T7)
+    st64, l2, _ = run(g.mesh, g.flow, "fp64", g.nrun)
+    st32, _, _ = run(g.mesh, g.flow, "fp32-mixed", g.nrun, history=False)
+    return g, st64, l2, st32
+
+
+@needs_ramp_t7
+def test_ramp_t7_20000_fp64_against_reference(ramp_t7_runs):
+    """Config 2's grid at its run length in FP64: the L2 history and every 97th
+    cell's final rho, u, v, p, nu~ against the reference's."""
+    g, st, l2, _ = ramp_t7_runs
+    idx = g["final_idx"]
+    dl2 = rel_dev(l2, g["hist_l2"].reshape(-1, 5)[:g.nrun])
+    errs = field_errors(st[:, idx], g["final"], g.flow)
+    print(f"{g.name}: L2 history {dl2:.2e}, sampled fields "
+          f"{({k: float('%.2e' % v) for k, v in errs.items()})}")
+    assert dl2 <= REL, dl2
+    assert max(errs.values()) <= REL, errs
+
+
+@needs_ramp_t7
+def test_ramp_t7_20000_fp32_mixed_against_reference(ramp_t7_runs):
+    """FP32-mixed at the ramp's run length against the reference's state (every
+    97th cell): pressure within 1e-5 relative L2 (the precision target); density,
+    u and nu~ are printed."""
+    from tdt import analysis
+    g, _, _, st32 = ramp_t7_runs
+    idx = g["final_idx"]
+    a = analysis.primitives_from_state(st32[:, idx], g.flow)
+    b = analysis.primitives_from_state(g["final"], g.flow)
+    errs = {k: analysis.rel_l2(a[k], b[k]) for k in ("p", "rho", "u", "nut")}
+    print("fp32-mixed vs reference after 20000 iterations (sampled):",
+          {k: float("%.2e" % v) for k, v in errs.items()})
+    assert errs["p"] <= 1e-5, errs
