@@ -388,12 +388,12 @@ def main():
     m, f = gridgen.WORKLOADS["ramp_2049x513"][0](), gridgen.WORKLOADS["ramp_2049x513"][1]()
     run_case("ramp_2049x513_t3", m, f, nrun=3720, dump=False, sample_stride=4099,
              crosscheck=False, big=True, workload="ramp_2049x513")
-    # the same ramp with the type-7 inflow over 20000 iterations, the run the
-    # FP32-mixed error at run length is stated on: L2 history and every 97th cell of
-    # the final state (about 3.3 hours on one core)
+    # the same ramp with the type-7 inflow over 5000 iterations (a quarter of the
+    # run the FP32-mixed error at run length is stated on; 20000 take about 11 hours
+    # on one core here): L2 history and every 97th cell of the final state
     m, f = (gridgen.WORKLOADS["ramp_2049x513_t7"][0](),
             gridgen.WORKLOADS["ramp_2049x513_t7"][1]())
-    run_case("ramp_2049x513_t7_20000", m, f, nrun=20000, dump=False, sample_stride=97,
+    run_case("ramp_2049x513_t7_5000", m, f, nrun=5000, dump=False, sample_stride=97,
              crosscheck=False, big=True, keep_linf=False, workload="ramp_2049x513_t7")
 
 
